@@ -1,0 +1,36 @@
+// Built-in operator object shared by the generic and fused paths.
+#pragma once
+
+#include <cstdint>
+
+#include "plan.h"
+#include "stepper.h"
+
+struct ffhip_op {
+  int kind = 0;
+  int prec = FFHIP_F64;
+  int ndim = 2;
+  int64_t n[3] = {1, 1, 1};   // nx, ny, nz
+  double box[3] = {1, 1, 1};  // Lx, Ly, Lz
+  double af = 1.0 / 3.0;      // aliased_fraction
+  double params[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int nparams = 0;
+  int64_t nkr = 1;
+  int64_t spec_count = 0, phys_count = 0;
+  int64_t lo[3] = {1, 1, 1}, hi[3] = {0, 0, 0};  // 1-based aliased ranges on the rfft layout
+  ffhip_plan* plan = nullptr;
+  void* spec[3] = {nullptr, nullptr, nullptr};
+  void* phys[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* tables = nullptr;  // tracer: sin/cos of the grid coordinates (FP64)
+  void* fused = nullptr;   // fused implementation state (power-of-two vorticity grids)
+};
+
+namespace ffhip {
+
+int op_calcN(ffhip_op* op, void* N, const void* sol, hipStream_t st);
+// fused path (vort2d_fused.hip): create sets op->fused when the grid qualifies
+int fused_create(ffhip_op* op);
+void fused_destroy(ffhip_op* op);
+int fused_etdrk4_run(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, hipStream_t st);
+
+}  // namespace ffhip

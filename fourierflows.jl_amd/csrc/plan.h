@@ -1,0 +1,39 @@
+// Host-side FFT plan objects shared by the transform code and the operators.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+
+namespace ffhip {
+
+// One transformed axis: complex FFT length, radix schedule, twiddle table
+// exp(-2 pi i k/n) (k < n) in the plan precision, and the execution path.
+struct Axis {
+  int64_t n = 1;
+  std::vector<int> rad;
+  void* tw = nullptr;
+  bool lds = false;  // whole transform in one LDS tile
+  int nb = 1;        // transforms (rows) or columns per block on the LDS path
+};
+
+void unit_root(int64_t k, int64_t n, double* c, double* s);
+std::vector<int> factor_radices(int64_t n);
+int upload_roots(int64_t n, int64_t count, int prec, void** dptr);
+
+}  // namespace ffhip
+
+struct ffhip_plan {
+  int kind = 0;  // 0 rfft, 1 fft
+  int ndim = 1;
+  int64_t n[3] = {1, 1, 1};
+  int64_t batch = 1;
+  int prec = FFHIP_F64;
+  ffhip::Axis ax[3];  // ax[0] is x; for rfft its length is nx/2
+  void* rtw = nullptr;  // rfft split twiddles exp(-2 pi i k/nx), k <= nx/2
+  void* ws = nullptr;   // spectral-size workspace (inverse keeps its input)
+  void* g0 = nullptr;   // global-path ping-pong buffers
+  void* g1 = nullptr;
+  void* g2 = nullptr;
+};

@@ -1,0 +1,41 @@
+"""fourierflows_hip -- FourierFlows.jl's pseudospectral step on AMD MI355X.
+
+A Python host layer over libffhip.so (HIP/C++ for gfx950, C ABI in
+include/ffhip.h) that keeps the reference's API names:
+
+* grids: ``OneDGrid``, ``TwoDGrid``, ``ThreeDGrid`` with ``rfftplan`` /
+  ``fftplan``, ``mul`` (``mul!``), ``ldiv`` (``ldiv!``), ``dealias``
+  (``dealias!``), ``getaliasedwavenumbers``        (src/domains.jl)
+* ``Equation``, ``Clock``, ``Problem``, ``TimeStepper``,
+  ``ETDRK4TimeStepper``, ``RK4TimeStepper``, ``stepforward``
+  (``stepforward!``), ``step_until``       (src/problem.jl, src/timesteppers.jl)
+* ``parsevalsum2``, ``parsevalsum``               (src/utils.jl)
+* built-in operators ``VorticityOperator`` / ``TracerOperator`` and the
+  ``Diffusion`` testbed                              (src/diffusion.jl)
+
+Device arrays are torch tensors on the ROCm device (``HIP()`` is the device
+tag, the analogue of ``GPU()``).
+"""
+
+from . import _lib
+from ._lib import DomainError, FFHIPError, NonFiniteError
+from .core import CPU, GPU, HIP, cxtype, device_array, fltype, init, zeros
+from .domains import (OneDGrid, Plan, ThreeDGrid, TwoDGrid, dealias, getaliasedwavenumbers, gridpoints, ldiv,
+                      mul)
+from .ics import splitmix64, uniform01, vorticity_ic
+from .physics import (Diffusion, TracerOperator, TracerProblem, TwoDVorticityProblem, VorticityOperator,
+                      vorticity_linear)
+from .timesteppers import (Clock, EmptyParams, EmptyVars, Equation, ETDRK4TimeStepper, Problem, RK4TimeStepper,
+                           TimeStepper, isexplicit, step_until, stepforward)
+from .utils import etdcoeffs, parsevalsum, parsevalsum2
+
+# Julia-style aliases
+stepforward_b = stepforward
+dealias_b = dealias
+
+__all__ = [n for n in dir() if not n.startswith("_")]
+
+
+def load_library():
+    """Load libffhip.so (raises if it has not been built)."""
+    return _lib.load()

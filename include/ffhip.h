@@ -1,0 +1,193 @@
+/*
+ * ffhip.h -- C ABI of libffhip.so, an MI355X (gfx950) implementation of
+ * FourierFlows.jl's pseudospectral step.
+ *
+ * Every entry point returns an int status (FFHIP_OK = 0, negative on error)
+ * and sets a thread-local message readable with ffhip_last_error().  No
+ * pointer passed in is retained after a call returns, except by the objects
+ * the create functions return.  Device pointers are plain HIP device
+ * addresses (from ffhip_malloc, hipMalloc or a framework allocator).  A
+ * `stream` argument of NULL means the HIP null stream.
+ *
+ * Array layout: Julia column-major with x fastest.  `dims[0]` is the x
+ * extent (nx for physical fields, nkr = nx/2+1 for rfft spectral fields),
+ * `dims[1]` is y, `dims[2]` is z.  Complex arrays are interleaved (re, im).
+ * `prec` is FFHIP_F32 (Float32 / ComplexF32) or FFHIP_F64.
+ *
+ * Reference interfaces replaced (FourierFlows.jl v0.10.7):
+ *   plan_flows_rfft / plan_flows_fft ........ src/domains.jl:2-5, :206-208, :347-349
+ *   mul!(fh, rfftplan, f) ................... src/domains.jl:5 (call: src/diffusion.jl:139)
+ *   ldiv!(f, rfftplan, fh) .................. src/diffusion.jl:137 (AbstractFFTs scaled inverse)
+ *   getaliasedwavenumbers / dealias! ........ src/domains.jl:408-421, :428-476
+ *   getexpLs / getetdcoeffs ................. src/timesteppers.jl:673-721
+ *   ETDRK4TimeStepper and its substeps ...... src/timesteppers.jl:434-463, :501-547
+ *   RK4TimeStepper and its substeps ......... src/timesteppers.jl:180-197, :225-274
+ *   parsevalsum2 / parsevalsum .............. src/utils.jl:113-183
+ *   stepforward!(prob, nsteps) .............. src/timesteppers.jl:6-20 (fused built-in operators)
+ *   zeros(dev, T, dims) / device_array ...... src/utils.jl:79-80, :329-332
+ */
+#ifndef FFHIP_H
+#define FFHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define FFHIP_OK 0
+#define FFHIP_ERR_DOMAIN (-1)      /* bad grid size (odd n): Julia DomainError, src/domains.jl:66,179,316 */
+#define FFHIP_ERR_ARG (-2)         /* bad argument (null pointer, bad precision code, bad rank) */
+#define FFHIP_ERR_OOM (-3)         /* device allocation failed */
+#define FFHIP_ERR_HIP (-4)         /* HIP runtime / launch failure */
+#define FFHIP_ERR_UNSUPPORTED (-5) /* valid request this build does not implement */
+#define FFHIP_ERR_NONFINITE (-6)   /* NaN/Inf found by a checked reduction */
+
+#define FFHIP_F32 32
+#define FFHIP_F64 64
+
+typedef struct ffhip_plan ffhip_plan;
+typedef struct ffhip_etdrk4 ffhip_etdrk4;
+typedef struct ffhip_rk4 ffhip_rk4;
+typedef struct ffhip_op ffhip_op;
+
+/* ---------------------------------------------------------------- runtime
+ * Device selection and memory: the HIPArray analogue of `zeros(dev,T,dims)`
+ * and `device_array(dev)` (src/utils.jl:79-80, :329-332). */
+int ffhip_init(int device);
+const char* ffhip_last_error(void);
+int ffhip_version(int* major, int* minor, int* patch);
+int ffhip_malloc(void** ptr, size_t bytes);            /* zero-filled, like zeros(dev, T, dims) */
+int ffhip_free(void* ptr);
+int ffhip_memcpy_htod(void* dst, const void* src, size_t bytes);
+int ffhip_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int ffhip_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+int ffhip_memset_zero(void* dst, size_t bytes, void* stream);
+int ffhip_synchronize(void);
+
+/* ---------------------------------------------------------------- FFT plans
+ * plan_flows_rfft (src/domains.jl:3,5) / plan_flows_fft (:2,4).
+ * n[0..ndim-1]: transform sizes, x first.  `batch` independent transforms
+ * are stored back to back.  Any even n (rfft x axis) / any n >= 1 (other
+ * axes) is accepted; factors 2,3,4,5,8 use unrolled butterflies, other
+ * primes a generic DFT stage. */
+int ffhip_plan_rfft(ffhip_plan** plan, int ndim, const int64_t* n, int64_t batch, int prec);
+int ffhip_plan_fft(ffhip_plan** plan, int ndim, const int64_t* n, int64_t batch, int prec);
+int ffhip_plan_destroy(ffhip_plan* plan);
+/* mul!(out, plan, in): unnormalised forward transform (sign -1).
+ * rfft: in real (n0,n1,..), out complex (n0/2+1, n1, ..).  In place is not allowed. */
+int ffhip_plan_forward(ffhip_plan* plan, const void* in, void* out, void* stream);
+/* ldiv!(out, plan, in): inverse transform scaled by 1/prod(n).  Unlike
+ * FFTW/CUFFT the input is preserved, so callers that deepcopy before ldiv!
+ * (src/diffusion.jl:158-159) are unaffected.  rfft: the imaginary parts of
+ * the x-DC and x-Nyquist entries are ignored after the y/z passes, as FFTW. */
+int ffhip_plan_inverse(ffhip_plan* plan, const void* in, void* out, void* stream);
+/* plan query: ndim, sizes, batch, precision, kind (0 = rfft, 1 = fft). */
+int ffhip_plan_info(const ffhip_plan* plan, int* ndim, int64_t* n, int64_t* batch, int* prec, int* kind);
+
+/* ---------------------------------------------------------------- dealiasing
+ * getaliasedwavenumbers (src/domains.jl:408-421), host only: 1-based
+ * inclusive index ranges; lo > hi means "nothing" (aliased_fraction == 0).
+ * Returns FFHIP_ERR_ARG if aliased_fraction >= 1 (:415). */
+int ffhip_aliased_wavenumbers(int64_t nk, int64_t nkr, double aliased_fraction,
+                              int64_t* kalias_lo, int64_t* kalias_hi,
+                              int64_t* kralias_lo, int64_t* kralias_hi);
+/* dealias!(fh, grid) (src/domains.jl:428-476): zero fh[lo[d]:hi[d]] along each
+ * dimension d (1-based inclusive ranges as above; lo > hi skips d). */
+int ffhip_dealias(void* fh, int prec, int ndim, const int64_t* dims, int64_t batch,
+                  const int64_t* lo, const int64_t* hi, void* stream);
+
+/* ---------------------------------------------------------------- ETDRK4
+ * getetdcoeffs (src/timesteppers.jl:689-718), host only, FP64: contour mean
+ * over ncirc points of radius rcirc.  L_im == NULL means real L (then only
+ * the real outputs are written; the *_im outputs may be NULL). */
+int ffhip_etdcoeffs(double dt, const double* L_re, const double* L_im, int64_t count,
+                    int ncirc, double rcirc,
+                    double* zeta_re, double* alpha_re, double* beta_re, double* gamma_re,
+                    double* zeta_im, double* alpha_im, double* beta_im, double* gamma_im);
+/* ETDRK4TimeStepper(equation, dt, dev) (src/timesteppers.jl:456-463).
+ * L (host, FP64, `count` = prod(dims) entries, already broadcast to dims).
+ * dt is first rounded to the state's float type (:457).  The coefficient
+ * arrays zeta, alpha, beta, Gamma, exp(dt L), exp(dt L/2) are kept in FP64 on
+ * the device; the scratch arrays sol1, sol2, N1..N4 are in `prec`. */
+int ffhip_etdrk4_create(ffhip_etdrk4** ts, const double* L_re, const double* L_im,
+                        int ndim, const int64_t* dims, double dt, int prec);
+int ffhip_etdrk4_destroy(ffhip_etdrk4* ts);
+/* which: 0 sol1, 1 sol2, 2 N1, 3 N2, 4 N3, 5 N4 (state-precision complex). */
+int ffhip_etdrk4_scratch(ffhip_etdrk4* ts, int which, void** ptr);
+/* which: 0 zeta, 1 alpha, 2 beta, 3 Gamma, 4 expLdt, 5 exp_half_Ldt.
+ * FP64, real (is_complex = 0) or complex interleaved (is_complex = 1). */
+int ffhip_etdrk4_coeff(ffhip_etdrk4* ts, int which, void** ptr, int* is_complex);
+/* ETDRK4substep12! (:506-510): out = exp_half_Ldt*sol + zeta*N */
+int ffhip_etdrk4_substep12(ffhip_etdrk4* ts, void* out, const void* sol, const void* N, void* stream);
+/* ETDRK4substep3! (:512-516): out = exp_half_Ldt*sol1 + zeta*(2*N3 - N1) */
+int ffhip_etdrk4_substep3(ffhip_etdrk4* ts, void* out, const void* sol1, const void* N1,
+                          const void* N3, void* stream);
+/* ETDRK4update! (:501-504): sol = expLdt*sol + alpha*N1 + 2*beta*(N2+N3) + Gamma*N4 */
+int ffhip_etdrk4_update(ffhip_etdrk4* ts, void* sol, const void* N1, const void* N2,
+                        const void* N3, const void* N4, void* stream);
+
+/* ---------------------------------------------------------------- RK4
+ * RK4TimeStepper (src/timesteppers.jl:180-197); L kept in FP64 on device. */
+int ffhip_rk4_create(ffhip_rk4** ts, const double* L_re, const double* L_im, int ndim,
+                     const int64_t* dims, int prec);
+int ffhip_rk4_destroy(ffhip_rk4* ts);
+/* which: 0 sol1, 1 RHS1, 2 RHS2, 3 RHS3, 4 RHS4 */
+int ffhip_rk4_scratch(ffhip_rk4* ts, int which, void** ptr);
+/* addlinearterm! (:225-229): RHS += L*sol */
+int ffhip_rk4_addlinearterm(ffhip_rk4* ts, void* RHS, const void* sol, void* stream);
+/* substepsol! (:231-235): newsol = sol + dt*RHS */
+int ffhip_rk4_substepsol(ffhip_rk4* ts, void* newsol, const void* sol, const void* RHS,
+                         double dt, void* stream);
+/* RK4update! (:260-264): sol += dt/6*(RHS1 + 2*RHS2 + 2*RHS3 + RHS4) */
+int ffhip_rk4_update(ffhip_rk4* ts, void* sol, const void* R1, const void* R2, const void* R3,
+                     const void* R4, double dt, void* stream);
+
+/* ---------------------------------------------------------------- reductions
+ * parsevalsum2 / parsevalsum (src/utils.jl:113-183), compensated FP64.
+ * rfft != 0: dims[0] == nkr and the kx = 0 and kx = nx/2 entries get weight 1,
+ * the rest weight 2 (:114-117).  The sum is multiplied by `norm`
+ * (Lx*Ly/(nx^2 ny^2) in 2D, :122).  Returns FFHIP_ERR_NONFINITE if the field
+ * holds NaN/Inf (the sum is still written). */
+int ffhip_parsevalsum2(const void* uh, int prec, int ndim, const int64_t* dims, int rfft,
+                       int64_t nx, double norm, double* out, void* stream);
+int ffhip_parsevalsum(const void* uh, int prec, int ndim, const int64_t* dims, int rfft,
+                      int64_t nx, double norm, double* out, void* stream);
+
+/* ---------------------------------------------------------------- built-in operators
+ * The reference's calcN! is arbitrary user Julia.  A C ABI cannot compile
+ * it, so the physics that the hot path needs is built in and selected by
+ * kind; each operator also runs whole fused time steps
+ * (stepforward!(prob, nsteps), src/timesteppers.jl:14-20).
+ *
+ * FFHIP_OP_VORTICITY2D: N = -J(psi, zeta), psi_hat = -zeta_hat/|k|^2, i.e.
+ *   N_hat = -i kr (u zeta)^ - i l (v zeta)^, u = -psi_y, v = psi_x, dealiased.
+ *   ndim = 2; params: none.
+ * FFHIP_OP_TRACER3D: N = -div(u c) for the steady ABC flow
+ *   u = (A sin z + C cos y, B sin x + A cos z, C sin y + B cos x), dealiased.
+ *   ndim = 3; params = {A, B, C, x0, y0, z0}.
+ * Both evaluate N on the dealiased state (modes dealias! would zero are
+ * read as zero) and return a dealiased N.  n: grid sizes, box: domain
+ * lengths (Lx, Ly[, Lz]). */
+#define FFHIP_OP_VORTICITY2D 1
+#define FFHIP_OP_TRACER3D 2
+int ffhip_op_create(ffhip_op** op, int kind, int prec, int ndim, const int64_t* n,
+                    const double* box, double aliased_fraction, const double* params, int nparams);
+int ffhip_op_destroy(ffhip_op* op);
+/* calcN!(N, sol, t, ...) for the built-in operator. */
+int ffhip_op_calcN(ffhip_op* op, void* N, const void* sol, double t, void* stream);
+/* nsteps fused ETDRK4 steps of `sol` (in place) with the coefficients of ts
+ * (created for this operator's grid and precision). */
+int ffhip_op_etdrk4_run(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, void* stream);
+/* nsteps fused RK4 steps with the linear term of ts and step dt. */
+int ffhip_op_rk4_run(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t nsteps, void* stream);
+/* Diagnostics on the rfft state: out[0] = energy, out[1] = enstrophy
+ * (vorticity: 1/2<|u|^2>, 1/2<zeta^2>; tracer: out[0] = 1/2<c^2>, out[1] = <c>). */
+int ffhip_op_diagnostics(ffhip_op* op, const void* sol, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FFHIP_H */

@@ -36,16 +36,8 @@ def uniform01(seed, n, offset=0):
     return (splitmix64(seed, n, offset) >> np.uint64(11)).astype(np.float64) * 2.0**-53
 
 
-def vorticity_ic(grid, seed, k0=6.0, rms=1.0, spectrum="k4exp1"):
-    """Random-phase ζ̂ with isotropic spectrum, scaled to rms ζ, dealiased.
-
-    spectrum "k4exp1": E(K) ∝ K⁴ exp(-(K/k0)²)      (256² and 2048² cases)
-    spectrum "k4exp2": E(K) ∝ K⁴ exp(-2 (K/k0)²)    (peak at K = k0; ensembles)
-
-    |ζ̂| ∝ sqrt(K E(K)); phases uniform from SplitMix64 over the rfft
-    half-plane in C order; then irfft -> scale to rms -> rfft -> dealias.
-    Returns complex128 spectral field of shape ``grid.spec_shape``.
-    """
+def vorticity_spectrum_amplitude(grid, seed, k0=6.0, spectrum="k4exp1"):
+    """Host FP64 random-phase ζ̂ before normalisation: |ζ̂| ∝ sqrt(K E(K))."""
     K = np.sqrt(grid.Krsq)
     if spectrum == "k4exp1":
         E = K**4 * np.exp(-((K / k0) ** 2))
@@ -57,6 +49,20 @@ def vorticity_ic(grid, seed, k0=6.0, rms=1.0, spectrum="k4exp1"):
     theta = 2 * np.pi * uniform01(seed, amp.size).reshape(amp.shape)
     zh = amp * np.exp(1j * theta)
     zh.flat[0] = 0
+    return zh
+
+
+def vorticity_ic(grid, seed, k0=6.0, rms=1.0, spectrum="k4exp1"):
+    """Random-phase ζ̂ with isotropic spectrum, scaled to rms ζ, dealiased.
+
+    spectrum "k4exp1": E(K) ∝ K⁴ exp(-(K/k0)²)      (256² and 2048² cases)
+    spectrum "k4exp2": E(K) ∝ K⁴ exp(-2 (K/k0)²)    (peak at K = k0; ensembles)
+
+    |ζ̂| ∝ sqrt(K E(K)); phases uniform from SplitMix64 over the rfft
+    half-plane in C order; then irfft -> scale to rms -> rfft -> dealias.
+    Returns complex128 spectral field of shape ``grid.spec_shape``.
+    """
+    zh = vorticity_spectrum_amplitude(grid, seed, k0, spectrum)
     z = grid.ldiv(zh)
     z *= rms / np.sqrt(np.mean(z**2))
     zh = grid.mul(z)

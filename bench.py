@@ -1,0 +1,203 @@
+"""Benchmark: ETDRK4 steps/s of 2D vorticity at 2048² on one MI355X.
+
+Workload (the headline configuration): 2D incompressible vorticity on
+[0, 2π)², nx = ny = 2048, hyperviscosity ν₈|k|⁸ with ν₈ set so the damping at
+2/3 of k_max is 1/dt, linear drag μ = 0.01, ETDRK4 with dt = 1e-3, seeded
+k⁴exp(-(k/6)²) initial vorticity at rms 1.  FP32 bulk with FP64 ETDRK4
+coefficients is the measured path (`value`); the same run in FP64 is reported
+beside it.
+
+A step is one ETDRK4 step of the whole 2048² field (4 nonlinear-term
+evaluations).  K steps are timed with HIP events after W warm-up steps, the
+state already resident in HBM.  The per-kernel roofline comes from a second,
+event-bracketed pass over the same kernels; the CPU baseline is the FP64
+NumPy/pocketfft restatement of the reference's algorithm timed on a bounded
+sample of steps on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "fourierflows.jl_amd"))
+
+METRIC = "ETDRK4 steps/s at 2048² FP32 vs FP64 on GPU; FFT HBM GB/s; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+N = 2048
+DT = 1e-3
+MU = 0.01
+
+
+def nu8_for(n, dt):
+    kcut = (n / 2) * (1 - 1 / 3)
+    return 1.0 / (dt * kcut**8)
+
+
+def make_problem(ff, T):
+    import numpy as np
+
+    return ff.TwoDVorticityProblem(nx=N, nu_n=nu8_for(N, DT), n_hyper=4, mu=MU, dt=DT,
+                                   T=np.float32 if T == 32 else np.float64)
+
+
+def time_steps(ff, torch, prob, steps, warmup):
+    ff.stepforward(prob, warmup)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    ff.stepforward(prob, steps)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1000.0
+
+
+def kernel_profile(ff, prob, steps):
+    """Per-kernel averages over `steps` event-bracketed steps."""
+    from fourierflows_hip import _lib
+
+    _lib.profile_start()
+    ff.stepforward(prob, steps)
+    rep = _lib.profile_stop()
+    out = {}
+    for k in rep["kernels"]:
+        avg_ms = k["total_ms"] / k["count"]
+        bytes_per = k["bytes"] / k["count"]
+        out[k["name"]] = {"count": k["count"], "avg_us": avg_ms * 1e3, "total_ms": k["total_ms"],
+                          "bytes_per_launch": bytes_per, "GBps": bytes_per / (avg_ms * 1e-3) / 1e9}
+    return out
+
+
+def cpu_baseline(seconds=12.0):
+    """FP64 oracle (NumPy + scipy.fft, reference operation order) on host cores."""
+    import numpy as np
+
+    workers = min(16, os.cpu_count() or 1)
+    os.environ["FFORACLE_WORKERS"] = str(workers)
+    import importlib
+
+    import oracle.grids as og
+    importlib.reload(og)
+    from oracle import ics as oics
+    from oracle import physics as oph
+    from oracle import timesteppers as ots
+
+    g = og.TwoDGrid(N, 2 * np.pi, x0=0, y0=0)
+    L = oph.vorticity_linear(g, nu_n=nu8_for(N, DT), n_hyper=4, mu=MU)
+    eqn = ots.Equation(L, oph.vorticity_calcN, g)
+    p = ots.Problem(eqn, "ETDRK4", DT, g)
+    p.sol[...] = oics.vorticity_ic(g, seed=1)
+    p.stepforward(1)  # warm pocketfft's plan cache
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds:
+        p.stepforward(1)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "steps/s", "cores": workers, "kind": "port",
+            "sample": f"{n} FP64 ETDRK4 steps of the same 2048² problem (NumPy/pocketfft, "
+                      f"{workers} FFT workers), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
+              "(the 2048² step is a single-GPU workload; ensembles shard without exchange)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import fourierflows_hip as ff
+
+    ff.init(0)
+    print(f"# bench: 2D vorticity {N}^2 ETDRK4, steps={a.steps} warmup={a.warmup}", file=sys.stderr, flush=True)
+
+    # FP32 (measured path)
+    p32 = make_problem(ff, 32)
+    if not p32.eqn.calcN.fused:
+        raise RuntimeError("fused path unavailable for the benchmark grid")
+    zh0 = ff.vorticity_ic(p32.grid, seed=1)
+    p32.sol.copy_(zh0)
+    t32 = time_steps(ff, torch, p32, a.steps, a.warmup)
+    sps32 = a.steps / t32
+    print(f"# FP32: {sps32:.1f} steps/s ({1e3 * t32 / a.steps:.4f} ms/step)", file=sys.stderr, flush=True)
+    prof32 = kernel_profile(ff, p32, a.profile_steps)
+
+    # FP64 on the same problem
+    k64 = a.fp64_steps or max(1, a.steps // 2)
+    p64 = make_problem(ff, 64)
+    zh64 = ff.vorticity_ic(p64.grid, seed=1)
+    p64.sol.copy_(zh64)
+    t64 = time_steps(ff, torch, p64, k64, max(1, a.warmup // 2))
+    sps64 = k64 / t64
+    print(f"# FP64: {sps64:.1f} steps/s", file=sys.stderr, flush=True)
+    prof64 = kernel_profile(ff, p64, a.profile_steps)
+
+    # accuracy of the FP32 run against the FP64 run from the same start, same step count
+    p32b = make_problem(ff, 32)
+    p32b.sol.copy_(zh64.to(torch.complex64))
+    p64b = make_problem(ff, 64)
+    p64b.sol.copy_(zh64)
+    nacc = min(a.steps, 500)
+    ff.stepforward(p32b, nacc)
+    ff.stepforward(p64b, nacc)
+    torch.cuda.synchronize()
+    x32 = p32b.sol.to(torch.complex128)
+    err = float(torch.linalg.vector_norm(x32 - p64b.sol) / torch.linalg.vector_norm(p64b.sol))
+    e32, z32 = p32b.eqn.calcN.energy_enstrophy(p32b.sol)
+    e64, z64 = p64b.eqn.calcN.energy_enstrophy(p64b.sol)
+
+    # dominant kernel of the FP32 step: the one with the largest total time
+    dom_name, dom = max(prof32.items(), key=lambda kv: kv[1]["total_ms"])
+    roofline = {"bound": "hbm", "achieved": round(dom["GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
+                "avg_us": round(dom["avg_us"], 3), "bytes_per_launch": dom["bytes_per_launch"]}
+    step_bytes = sum(v["bytes_per_launch"] * v["count"] for v in prof32.values()) / a.profile_steps
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(a.cpu_seconds)
+
+    line = {
+        "metric": METRIC, "value": round(sps32, 2), "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * t32 / a.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 ETDRK4 coefficients)",
+        "data": "synthetic (seeded k^4 exp(-(k/6)^2) vorticity, rms 1)",
+        "config": {"workload": "2D vorticity 2048^2 ETDRK4, nu8|k|^8 + mu=0.01, dt=1e-3, 2/3 dealiasing",
+                   "nx": N, "ny": N, "stepper": "ETDRK4", "dt": DT, "parallelism": "single GPU"},
+        "fp64_steps_per_s": round(sps64, 2), "fp32_over_fp64": round(sps32 / sps64, 3),
+        "fp32_rel_l2_vs_fp64": {"steps": nacc, "field": err, "energy": abs(e32 - e64) / abs(e64),
+                                "enstrophy": abs(z32 - z64) / abs(z64)},
+        "step_algorithmic_GB": round(step_bytes / 1e9, 4),
+        "step_GBps": round(step_bytes / (t32 / a.steps) / 1e9, 1),
+        "roofline": roofline,
+        "kernels_fp32": {k: {kk: round(vv, 3) if isinstance(vv, float) else vv for kk, vv in v.items()}
+                         for k, v in prof32.items()},
+        "kernels_fp64": {k: {kk: round(vv, 3) if isinstance(vv, float) else vv for kk, vv in v.items()}
+                         for k, v in prof64.items()},
+        "cpu_baseline": cpu,
+        "reference_published_gpu_steps_per_s_2048": 18.3,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

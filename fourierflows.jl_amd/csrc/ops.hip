@@ -439,6 +439,13 @@ int ffhip_op_rk4_run(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t 
   return rk4_run_generic(op, ts, sol, h, nsteps, as_stream(stream));
 }
 
+int ffhip_op_is_fused(const ffhip_op* op, int* fused) {
+  clear_error();
+  FF_CHECK(op && fused, FFHIP_ERR_ARG, "null argument");
+  *fused = op->fused != nullptr;
+  return FFHIP_OK;
+}
+
 int ffhip_op_diagnostics(ffhip_op* op, const void* sol, double* out, void* stream) {
   clear_error();
   FF_CHECK(op && sol && out, FFHIP_ERR_ARG, "null argument");

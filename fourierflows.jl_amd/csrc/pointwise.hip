@@ -52,8 +52,9 @@ template <typename R, bool CC>
 __global__ void k_etd_sub3(cplx<R>* out, const cplx<R>* sol1, const cplx<R>* N1, const cplx<R>* N3, const void* Eh,
                            const void* Z, int64_t n) {
   FF_GRID_LOOP(i, n) {
-    const double2 n1 = ld2<R>(N1, i), n3 = ld2<R>(N3, i);
-    const double2 d = make_double2(2 * n3.x - n1.x, 2 * n3.y - n1.y);
+    // 2N3 - N1 is formed in the state type, as the reference's broadcast does
+    const cplx<R> n1 = N1[i], n3 = N3[i];
+    const double2 d = make_double2((double)(R(2) * n3.x - n1.x), (double)(R(2) * n3.y - n1.y));
     st2<R>(out, i, d2add(cmulz<CC>(Eh, i, ld2<R>(sol1, i)), cmulz<CC>(Z, i, d)));
   }
 }
@@ -65,7 +66,9 @@ __global__ void k_etd_update(cplx<R>* sol, const cplx<R>* N1, const cplx<R>* N2,
                              int64_t n) {
   FF_GRID_LOOP(i, n) {
     double2 acc = d2add(cmulz<CC>(E, i, ld2<R>(sol, i)), cmulz<CC>(A, i, ld2<R>(N1, i)));
-    const double2 s23 = d2add(ld2<R>(N2, i), ld2<R>(N3, i));
+    // N2 + N3 is formed in the state type, as the reference's broadcast does
+    const cplx<R> n2 = N2[i], n3 = N3[i];
+    const double2 s23 = make_double2((double)(n2.x + n3.x), (double)(n2.y + n3.y));
     acc = d2add(acc, d2scale(cmulz<CC>(B, i, s23), 2.0));
     acc = d2add(acc, cmulz<CC>(G, i, ld2<R>(N4, i)));
     st2<R>(sol, i, acc);

@@ -8,8 +8,41 @@
 #include <vector>
 
 #include "common.h"
+#include "prof.h"
 
 namespace ffhip {
+
+// ----------------------------------------------------------------- profiler
+bool g_prof_on = false;
+struct ProfRec {
+  const char* name;
+  double bytes;
+  hipEvent_t a, b;
+};
+static std::vector<ProfRec> g_prof;
+static std::vector<hipEvent_t> g_evpool;
+static size_t g_evnext = 0;
+
+static hipEvent_t prof_event() {
+  if (g_evnext == g_evpool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    g_evpool.push_back(e);
+  }
+  return g_evpool[g_evnext++];
+}
+
+void prof_begin_impl(const char* name, double bytes, hipStream_t st) {
+  ProfRec r{name, bytes, prof_event(), prof_event()};
+  if (r.a && r.b) {
+    (void)hipEventRecord(r.a, st);
+    g_prof.push_back(r);
+  }
+}
+
+void prof_end_impl(hipStream_t st) {
+  if (!g_prof.empty()) (void)hipEventRecord(g_prof.back().b, st);
+}
 
 static thread_local std::string g_err;
 
@@ -148,6 +181,56 @@ int ffhip_memset_zero(void* dst, size_t bytes, void* stream) {
 int ffhip_synchronize(void) {
   clear_error();
   FF_HIP(hipDeviceSynchronize());
+  return FFHIP_OK;
+}
+
+int ffhip_profile_start(void) {
+  clear_error();
+  g_prof.clear();
+  g_evnext = 0;
+  g_prof_on = true;
+  return FFHIP_OK;
+}
+
+int ffhip_profile_stop(char* buf, size_t len) {
+  clear_error();
+  g_prof_on = false;
+  FF_HIP(hipDeviceSynchronize());
+  struct Agg {
+    std::string name;
+    long count = 0;
+    double ms = 0, bytes = 0;
+  };
+  std::vector<Agg> agg;
+  for (const ProfRec& r : g_prof) {
+    float ms = 0;
+    FF_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    Agg* a = nullptr;
+    for (auto& x : agg)
+      if (x.name == r.name) a = &x;
+    if (!a) {
+      agg.push_back(Agg());
+      a = &agg.back();
+      a->name = r.name;
+    }
+    a->count += 1;
+    a->ms += ms;
+    a->bytes += r.bytes;
+  }
+  std::string js = "{\"kernels\": [";
+  for (size_t i = 0; i < agg.size(); ++i) {
+    char tmp[512];
+    snprintf(tmp, sizeof tmp, "%s{\"name\": \"%s\", \"count\": %ld, \"total_ms\": %.6f, \"bytes\": %.1f}",
+             i ? ", " : "", agg[i].name.c_str(), agg[i].count, agg[i].ms, agg[i].bytes);
+    js += tmp;
+  }
+  js += "]}";
+  g_prof.clear();
+  g_evnext = 0;
+  if (buf && len) {
+    snprintf(buf, len, "%s", js.c_str());
+    FF_CHECK(js.size() < len, FFHIP_ERR_ARG, "profile buffer too small (%zu bytes needed)", js.size() + 1);
+  }
   return FFHIP_OK;
 }
 

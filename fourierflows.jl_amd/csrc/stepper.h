@@ -20,6 +20,7 @@ struct ffhip_etdrk4 {
   // operator-private compacted copies of the coefficients (fused paths)
   void* packed = nullptr;
   const void* packed_owner = nullptr;
+  uint64_t packed_key = 0;  // geometry the packed copy was made for
   // host copy of the linear operator (FP64) for the operators that need it
   double* L_host_re = nullptr;
   double* L_host_im = nullptr;

@@ -1,0 +1,127 @@
+// Register-resident Stockham FFT of a power-of-two length N by a "team" of
+// T = N/P threads, each holding P points.
+//
+// On entry thread `tid` holds x[tid + p*T] in v[p]; on exit it holds
+// X[tid + p*T].  Passes use radix P (and one smaller radix for the
+// remainder); the first pass reads registers, the last writes registers, the
+// passes in between exchange through a per-team LDS buffer of N + N/16
+// complex values (one pad slot per 16 keeps strided writes off a single
+// bank).  The caller's block runs all teams through the same sequence, so
+// __syncthreads() is the team barrier.
+#pragma once
+
+#include "fft_device.h"
+
+namespace ffhip {
+
+template <int DIR, typename R> struct Bfly<16, DIR, R> {
+  static __device__ __forceinline__ void run(cplx<R>* v) {
+    // 16 = 4 x 4: DFT4 over n1 for each n2, twiddle W16^(n2 k1), DFT4 over n2
+    cplx<R> a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      cplx<R> t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+      Bfly<4, DIR, R>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
+    }
+    const R c1 = R(0.92387953251128675612818318939678829);  // cos(pi/8)
+    const R s1 = R(0.38268343236508977172845998403039887);  // sin(pi/8)
+    const R h = R(0.70710678118654752440084436210484904);
+    // W16^m = exp(DIR*2*pi*i*m/16) for m = n2*k1 in {1,2,3,4,6,9}
+    auto rot = [&](cplx<R> z, R c, R s) { return mkc<R>(z.x * c - DIR * z.y * s, z.y * c + DIR * z.x * s); };
+    a[1][1] = rot(a[1][1], c1, s1);
+    a[1][2] = rot(a[1][2], h, h);
+    a[1][3] = rot(a[1][3], s1, c1);
+    a[2][1] = rot(a[2][1], h, h);
+    a[2][2] = cmul_si<DIR, R>(a[2][2]);
+    a[2][3] = rot(a[2][3], -h, h);
+    a[3][1] = rot(a[3][1], s1, c1);
+    a[3][2] = rot(a[3][2], -h, h);
+    a[3][3] = rot(a[3][3], -c1, -s1);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      cplx<R> t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+      Bfly<4, DIR, R>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  }
+};
+
+constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
+constexpr int cfull(int N, int P) { return (N % P == 0 && N >= P) ? 1 + cfull(N / P, P) : 0; }
+
+template <int N, int P> struct TeamSched {
+  static_assert((N & (N - 1)) == 0 && (P & (P - 1)) == 0, "powers of two only");
+  static constexpr int T = N / P;
+  static constexpr int NFULL = cfull(N, P);
+  static constexpr int REM = N / cpow(P, NFULL);
+  static constexpr int NPASS = NFULL + (REM > 1 ? 1 : 0);
+  static constexpr int LDS = N + N / 16;  // complex slots per team
+};
+
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+
+template <int N, int P, int RK, int NS, bool FIRST, bool LAST, int DIR, typename R>
+__device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__ lds, int tid,
+                                          const cplx<R>* __restrict__ tw) {
+  constexpr int T = N / P;
+  constexpr int NB = P / RK;  // butterflies per thread
+  constexpr int M = N / RK;
+  cplx<R> x[P];
+  if constexpr (FIRST) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int r = 0; r < RK; ++r) x[b * RK + r] = v[b + r * NB];
+  } else {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int r = 0; r < RK; ++r) x[b * RK + r] = lds[lpad(tid + b * T + r * M)];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = tid + b * T;
+    const int k = j & (NS - 1);
+    cplx<R>* y = x + b * RK;
+    if constexpr (NS > 1) {
+      constexpr int TS = N / (NS * RK);
+#pragma unroll
+      for (int r = 1; r < RK; ++r) y[r] = cmul<R>(y[r], twid<DIR, R>(tw[r * k * TS]));
+    }
+    Bfly<RK, DIR, R>::run(y);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < RK; ++r) v[b + r * NB] = y[r];
+    } else {
+      const int o = (j - k) * RK + k;
+#pragma unroll
+      for (int r = 0; r < RK; ++r) lds[lpad(o + r * NS)] = y[r];
+    }
+  }
+  if constexpr (!LAST) __syncthreads();
+}
+
+template <int N, int P, int S, int DIR, typename R>
+__device__ __forceinline__ void team_passes(cplx<R> (&v)[P], cplx<R>* lds, int tid, const cplx<R>* tw) {
+  using TS = TeamSched<N, P>;
+  if constexpr (S < TS::NPASS) {
+    constexpr int RK = S < TS::NFULL ? P : TS::REM;
+    constexpr int NS = cpow(P, S);
+    team_pass<N, P, RK, NS, S == 0, S == TS::NPASS - 1, DIR, R>(v, lds, tid, tw);
+    team_passes<N, P, S + 1, DIR, R>(v, lds, tid, tw);
+  }
+}
+
+// Full transform.  Requires NPASS >= 2 (N >= 2P) so that the first pass
+// writes LDS and the last pass reads it.  Ends with LDS free for reuse.
+template <int N, int P, int DIR, typename R>
+__device__ __forceinline__ void team_fft(cplx<R> (&v)[P], cplx<R>* lds, int tid, const cplx<R>* tw) {
+  static_assert(TeamSched<N, P>::NPASS >= 2, "team_fft needs at least two passes");
+  team_passes<N, P, 0, DIR, R>(v, lds, tid, tw);
+}
+
+}  // namespace ffhip

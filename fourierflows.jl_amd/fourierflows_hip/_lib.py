@@ -93,6 +93,9 @@ SIGNATURES = {
     "ffhip_op_etdrk4_run": [_vp, _vp, _vp, _i64, _vp],
     "ffhip_op_rk4_run": [_vp, _vp, _vp, _dbl, _i64, _vp],
     "ffhip_op_diagnostics": [_vp, _vp, _dp, _vp],
+    "ffhip_op_is_fused": [_vp, C.POINTER(_int)],
+    "ffhip_profile_start": [],
+    "ffhip_profile_stop": [C.c_char_p, C.c_size_t],
 }
 _RESTYPE = {"ffhip_last_error": C.c_char_p}
 
@@ -145,6 +148,19 @@ def call(name, *args):
 def i64arr(vals):
     vals = list(vals)
     return (C.c_int64 * max(1, len(vals)))(*vals)
+
+
+def profile_start():
+    call("ffhip_profile_start")
+
+
+def profile_stop():
+    """Per-kernel event timings since profile_start() (dict from the JSON report)."""
+    import json
+
+    buf = C.create_string_buffer(1 << 16)
+    call("ffhip_profile_stop", buf, len(buf))
+    return json.loads(buf.value.decode())
 
 
 def dptr(a):

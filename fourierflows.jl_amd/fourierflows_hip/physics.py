@@ -75,6 +75,13 @@ class VorticityOperator(BuiltinOperator):
             raise TypeError("VorticityOperator needs a TwoDGrid")
         self._create(grid)
 
+    @property
+    def fused(self):
+        """True when libffhip runs this grid through the fused kernels."""
+        v = C.c_int()
+        _lib.call("ffhip_op_is_fused", self._h, C.byref(v))
+        return bool(v.value)
+
     def energy_enstrophy(self, sol):
         """(½⟨|u|²⟩, ½⟨ζ²⟩) by compensated FP64 Parseval sums."""
         return self.diagnostics(sol)

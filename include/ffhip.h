@@ -187,6 +187,18 @@ int ffhip_op_rk4_run(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t 
  * (vorticity: 1/2<|u|^2>, 1/2<zeta^2>; tracer: out[0] = 1/2<c^2>, out[1] = <c>). */
 int ffhip_op_diagnostics(ffhip_op* op, const void* sol, double* out, void* stream);
 
+/* Is the fused path active for this operator (power-of-two 2D vorticity grid)? */
+int ffhip_op_is_fused(const ffhip_op* op, int* fused);
+
+/* ---------------------------------------------------------------- profiling
+ * No counterpart in the reference.  Between start and stop the fused step
+ * brackets every kernel launch with HIP events on its stream; stop
+ * synchronises and writes a JSON summary {"kernels": [{"name", "count",
+ * "total_ms", "bytes"}]} where bytes are the algorithmic bytes each launch
+ * must move (summed over launches). */
+int ffhip_profile_start(void);
+int ffhip_profile_stop(char* buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,10 +34,36 @@
 
 namespace ffhip {
 
-constexpr int kVCB = 4;  // kx columns per column block
-constexpr int kVRR = 4;  // y rows per row block
+#ifndef FF_STAGE_UNROLL
+#define FF_STAGE_UNROLL 4  // stage-combination loads in flight per thread (register budget)
+#endif
 
-constexpr int vpts(int n) { return n >= 256 ? 16 : 8; }
+constexpr int kVCB = 4;  // kx columns per layout block (tile width)
+#ifndef FF_CBK
+#define FF_CBK 2  // kx columns per column-kernel workgroup (a sub-block of the tile)
+#endif
+#ifndef FF_RRK
+#define FF_RRK 2  // y rows per row-kernel workgroup
+#endif
+constexpr int kCBK = FF_CBK;
+constexpr int kRRK = FF_RRK;
+constexpr int kCSUB = kVCB / kCBK;  // column workgroups sharing one tile
+constexpr int kRSUB = 4 / kRRK;     // row workgroups sharing the 4-row, 128 B chunks
+constexpr int kXCD = 8;             // workgroups are dispatched round-robin over 8 XCDs
+
+// Workgroups that share cache lines are placed on the same XCD (same L2):
+// block b runs on XCD b % 8, so sub-block s of group q gets
+// b = (q % 8) + 8 * ((q / 8) * NSUB + s).
+__device__ __forceinline__ void xcd_split(int b, int nsub, int* group, int* sub) {
+  const int x = b % kXCD;
+  const int slot = b / kXCD;
+  *sub = slot % nsub;
+  *group = (slot / nsub) * kXCD + x;
+}
+
+// points per thread of the y (column) and x (row) transforms
+constexpr int vpts_col(int n) { return n >= 256 ? 16 : 8; }
+constexpr int vpts_row(int n) { return n >= 256 ? 16 : 8; }
 
 struct VortGeom {
   int nx, ny, nkr;
@@ -87,16 +113,19 @@ struct VortColArgs {
 
 // STAGE 0: y-inverse of U only.  STAGE 1..4: ETDRK4 stages.
 template <typename R, int NY, int P, int STAGE>
-__global__ void __launch_bounds__(kVCB*(NY / P)) k_vcol(VortGeom g, VortColArgs a) {
+__global__ void __launch_bounds__(kCBK*(NY / P)) k_vcol(VortGeom g, VortColArgs a) {
   constexpr int T = NY / P;
   constexpr int CB = kVCB;
   extern __shared__ unsigned char vsm[];
+  int tx, sub;
+  xcd_split(blockIdx.x, kCSUB, &tx, &sub);
+  if (tx >= g.TX) return;  // grid is rounded up to whole XCD groups
   const int t = threadIdx.x;
-  const int c = t % CB;
-  const int tid = t / CB;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + c * TeamSched<NY, P>::LDS;
+  const int cc = t % kCBK;
+  const int c = sub * kCBK + cc;
+  const int tid = t / kCBK;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + cc * TeamSched<NY, P>::LDS;
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
-  const int tx = blockIdx.x;
   const int ix = tx * CB + c;
   const bool colkept = ix < g.KX;
   const R kx = (R)(g.dk * (double)ix);
@@ -129,7 +158,7 @@ __global__ void __launch_bounds__(kVCB*(NY / P)) k_vcol(VortGeom g, VortColArgs 
   cplx<R>* N1 = reinterpret_cast<cplx<R>*>(a.N1);
   cplx<R>* N2 = reinterpret_cast<cplx<R>*>(a.N2);
   cplx<R>* As = reinterpret_cast<cplx<R>*>(a.A);
-#pragma unroll
+#pragma unroll FF_STAGE_UNROLL
   for (int p = 0; p < P; ++p) {
     const int iy = tid + p * T;
     const bool kept = colkept && (iy < g.ylo || iy > g.yhi);
@@ -221,17 +250,19 @@ struct VortRowArgs {
 };
 
 template <typename R, int NX, int P>
-__global__ void __launch_bounds__(kVRR*(NX / P)) k_vrow(VortGeom g, VortRowArgs a) {
+__global__ void __launch_bounds__(kRRK*(NX / P)) k_vrow(VortGeom g, VortRowArgs a) {
   constexpr int T = NX / P;
   constexpr int CB = kVCB;
-  constexpr int RR = kVRR;
   extern __shared__ unsigned char vsm[];
+  int grp, sub;
+  xcd_split(blockIdx.x, kRSUB, &grp, &sub);
+  if (grp >= g.ny / 4) return;  // grid is rounded up to whole XCD groups
   const int t = threadIdx.x;
-  const int r = t % RR;
-  const int tid = t / RR;
+  const int r = t % kRRK;
+  const int tid = t / kRRK;
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + r * TeamSched<NX, P>::LDS;
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
-  const int y = blockIdx.x * RR + r;
+  const int y = grp * 4 + sub * kRRK + r;
   const int64_t ny = g.ny;
   auto iaddr = [&](int k) -> int64_t { return ((int64_t)(k / CB) * ny + y) * CB + (k % CB); };
   const cplx<R>* Zh = reinterpret_cast<const cplx<R>*>(a.I0);
@@ -365,12 +396,12 @@ static bool pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 template <typename R, int N>
 static size_t col_lds() {
-  return (size_t)kVCB * TeamSched<N, vpts(N)>::LDS * sizeof(cplx<R>);
+  return (size_t)kCBK * TeamSched<N, vpts_col(N)>::LDS * sizeof(cplx<R>);
 }
 
 template <typename R, int N>
 static size_t row_lds() {
-  return (size_t)kVRR * TeamSched<N, vpts(N)>::LDS * sizeof(cplx<R>);
+  return (size_t)kRRK * TeamSched<N, vpts_row(N)>::LDS * sizeof(cplx<R>);
 }
 
 // Algorithmic bytes of one launch: only the kept band (KX columns, KY kept
@@ -397,7 +428,7 @@ static double row_bytes(const VortGeom& g) {
 
 template <typename R, int N, int STAGE>
 static int launch_col(const VortGeom& g, const VortColArgs& a, hipStream_t st) {
-  constexpr int P = vpts(N);
+  constexpr int P = vpts_col(N);
   const size_t sh = col_lds<R, N>();
   auto kern = k_vcol<R, N, P, STAGE>;
   static bool attr_set = false;
@@ -409,7 +440,8 @@ static int launch_col(const VortGeom& g, const VortColArgs& a, hipStream_t st) {
   static const char* names[5] = {"vort_col_stage0", "vort_col_stage1", "vort_col_stage2", "vort_col_stage3",
                                  "vort_col_stage4"};
   prof_begin(names[STAGE], col_bytes<R>(g, STAGE, a.do_inverse), st);
-  hipLaunchKernelGGL(kern, dim3(g.TX), dim3(kVCB * (N / P)), sh, st, g, a);
+  const int groups = (g.TX + kXCD - 1) / kXCD * kXCD;
+  hipLaunchKernelGGL(kern, dim3(groups * kCSUB), dim3(kCBK * (N / P)), sh, st, g, a);
   prof_end(st);
   FF_LAUNCHED("k_vcol");
   return FFHIP_OK;
@@ -417,7 +449,7 @@ static int launch_col(const VortGeom& g, const VortColArgs& a, hipStream_t st) {
 
 template <typename R, int N>
 static int launch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
-  constexpr int P = vpts(N);
+  constexpr int P = vpts_row(N);
   const size_t sh = row_lds<R, N>();
   auto kern = k_vrow<R, N, P>;
   static bool attr_set = false;
@@ -427,7 +459,8 @@ static int launch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
     attr_set = true;
   }
   prof_begin("vort_row", row_bytes<R>(g), st);
-  hipLaunchKernelGGL(kern, dim3(g.ny / kVRR), dim3(kVRR * (N / P)), sh, st, g, a);
+  const int groups = (g.ny / 4 + kXCD - 1) / kXCD * kXCD;
+  hipLaunchKernelGGL(kern, dim3(groups * kRSUB), dim3(kRRK * (N / P)), sh, st, g, a);
   prof_end(st);
   FF_LAUNCHED("k_vrow");
   return FFHIP_OK;

@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -128,6 +130,13 @@ def main():
     import fourierflows_hip as ff
 
     ff.init(0)
+    if a.calib_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     print(f"# bench: 2D vorticity {N}^2 ETDRK4, steps={a.steps} warmup={a.warmup}", file=sys.stderr, flush=True)
 
     # FP32 (measured path)

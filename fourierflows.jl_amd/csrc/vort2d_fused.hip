@@ -34,8 +34,20 @@
 
 namespace ffhip {
 
+#ifndef FF_COL_PREFETCH
+#define FF_COL_PREFETCH 0
+#endif
+#ifndef FF_ROW_PREFETCH_F32
+#define FF_ROW_PREFETCH_F32 0
+#endif
+#ifndef FF_ROW_PREFETCH_F64
+#define FF_ROW_PREFETCH_F64 1
+#endif
+#ifndef FF_ROW_ZETA_FIRST
+#define FF_ROW_ZETA_FIRST 0
+#endif
 #ifndef FF_STAGE_UNROLL
-#define FF_STAGE_UNROLL 4  // stage-combination loads in flight per thread (register budget)
+#define FF_STAGE_UNROLL 16  // stage-combination loads in flight per thread
 #endif
 
 constexpr int kVCB = 4;  // kx columns per layout block (tile width)
@@ -139,9 +151,17 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vcol(VortGeom g, VortColArgs 
     const cplx<R>* O1 = reinterpret_cast<const cplx<R>*>(a.O1);
 #pragma unroll
     for (int p = 0; p < P; ++p) nn[p] = O0[ibase + (int64_t)(tid + p * T) * CB];
+#if FF_COL_PREFETCH
+    // issue the second array's loads before the first transform so their
+    // latency hides behind its arithmetic
+#pragma unroll
+    for (int p = 0; p < P; ++p) vb[p] = O1[ibase + (int64_t)(tid + p * T) * CB];
+    team_fft<NY, P, -1, R>(nn, lds, tid, tw);
+#else
     team_fft<NY, P, -1, R>(nn, lds, tid, tw);
 #pragma unroll
     for (int p = 0; p < P; ++p) vb[p] = O1[ibase + (int64_t)(tid + p * T) * CB];
+#endif
     team_fft<NY, P, -1, R>(vb, lds, tid, tw);
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -270,6 +290,51 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_vrow(VortGeom g, VortRowArgs 
   const cplx<R>* Vh = reinterpret_cast<const cplx<R>*>(a.I2);
   const int KX = g.KX;
 
+#if FF_ROW_ZETA_FIRST
+  // zeta first: only its real part (P values) stays live across the u + i v transform
+  R zr[P];
+  {
+    cplx<R> w[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int k = tid + p * T;
+      cplx<R> s = mkc<R>(0, 0);
+      if (k == 0) {
+        s = mkc<R>(Zh[iaddr(0)].x, 0);
+      } else if (k < KX) {
+        s = Zh[iaddr(k)];
+      } else if (k > NX - KX) {
+        s = cconj<R>(Zh[iaddr(NX - k)]);
+      }
+      w[p] = s;
+    }
+    team_fft<NX, P, 1, R>(w, lds, tid, tw);
+    const R s2 = (R)g.s2;
+#pragma unroll
+    for (int p = 0; p < P; ++p) zr[p] = w[p].x * s2;
+  }
+  cplx<R> w[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int k = tid + p * T;
+    cplx<R> s = mkc<R>(0, 0);
+    if (k == 0) {
+      const cplx<R> U = Uh[iaddr(0)], V = Vh[iaddr(0)];
+      s = mkc<R>(U.x, V.x);
+    } else if (k < KX) {
+      const cplx<R> U = Uh[iaddr(k)], V = Vh[iaddr(k)];
+      s = mkc<R>(U.x - V.y, U.y + V.x);
+    } else if (k > NX - KX) {
+      const int k2 = NX - k;
+      const cplx<R> U = Uh[iaddr(k2)], V = Vh[iaddr(k2)];
+      s = mkc<R>(U.x + V.y, V.x - U.y);
+    }
+    w[p] = s;
+  }
+  team_fft<NX, P, 1, R>(w, lds, tid, tw);
+#pragma unroll
+  for (int p = 0; p < P; ++p) w[p] = mkc<R>(w[p].x * zr[p], w[p].y * zr[p]);
+#else
   // u + i v: Hermitian-extended half spectra packed into one complex inverse
   cplx<R> z[P];
 #pragma unroll
@@ -289,7 +354,10 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_vrow(VortGeom g, VortRowArgs 
     }
     z[p] = s;
   }
-  team_fft<NX, P, 1, R>(z, lds, tid, tw);
+  // FP64: issue the zeta^ loads before the u + i v transform (measured faster);
+  // FP32: transform first (the extra live registers cost more than they hide)
+  constexpr bool prefetch = sizeof(R) == 8 ? FF_ROW_PREFETCH_F64 : FF_ROW_PREFETCH_F32;
+  if constexpr (!prefetch) team_fft<NX, P, 1, R>(z, lds, tid, tw);
 
   cplx<R> w[P];
 #pragma unroll
@@ -305,6 +373,7 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_vrow(VortGeom g, VortRowArgs 
     }
     w[p] = s;
   }
+  if constexpr (prefetch) team_fft<NX, P, 1, R>(z, lds, tid, tw);
   team_fft<NX, P, 1, R>(w, lds, tid, tw);
 
   // products u zeta + i v zeta with the 1/(nx ny)^2 normalisation
@@ -314,6 +383,7 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_vrow(VortGeom g, VortRowArgs 
     const R zz = w[p].x * s2;
     w[p] = mkc<R>(z[p].x * zz, z[p].y * zz);
   }
+#endif
   team_fft<NX, P, -1, R>(w, lds, tid, tw);
 
   // split W into the half spectra of its real and imaginary parts

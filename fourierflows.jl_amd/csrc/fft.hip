@@ -506,8 +506,9 @@ static int exec_forward(ffhip_plan* p, const void* vin, void* vout, hipStream_t 
   return FFHIP_OK;
 }
 
+// extra: factor on top of the 1/prod(n) normalisation (prod(n) gives the unnormalised inverse)
 template <typename R>
-static int exec_inverse(ffhip_plan* p, const void* vin, void* vout, hipStream_t st) {
+static int exec_inverse(ffhip_plan* p, const void* vin, void* vout, hipStream_t st, double extra = 1.0) {
   const int64_t* n = p->n;
   const int nd = p->ndim;
   const int64_t nx0 = p->kind == 0 ? n[0] / 2 + 1 : n[0];
@@ -529,7 +530,7 @@ static int exec_inverse(ffhip_plan* p, const void* vin, void* vout, hipStream_t 
                              st));
       cur = ws;
     }
-    return run_rows_c2c<R>(cur, out, rest, p->ax[0], 1, R(1.0 / total), p, st);
+    return run_rows_c2c<R>(cur, out, rest, p->ax[0], 1, R(extra / total), p, st);
   }
   if (nd >= 3) {
     FF_TRY(run_cols_c2c<R>(cur, ws, nx0 * n[1], nx0 * n[1], p->batch, nx0 * n[1] * n[2], p->ax[2], 1, R(1), p,
@@ -546,7 +547,59 @@ static int exec_inverse(ffhip_plan* p, const void* vin, void* vout, hipStream_t 
   // the other axes contribute 1/prod(n[1:]).
   const double m = (double)(n[0] / 2);
   const double other = total / (double)n[0];
-  return run_rows_c2r<R>(cur, reinterpret_cast<R*>(vout), rest, R(1.0 / (m * other)), p, st);
+  return run_rows_c2r<R>(cur, reinterpret_cast<R*>(vout), rest, R(extra / (m * other)), p, st);
+}
+
+// out = in * (edge on the kx = 0 and kx = nx/2 columns, mid elsewhere), rfft layout [rest][nx0]
+template <typename R>
+__global__ void k_rfft_weight(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out, int64_t count,
+                              int64_t nx0, R edge, R mid) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t kx = i % nx0;
+    const R s = (kx == 0 || kx == nx0 - 1) ? edge : mid;
+    const cplx<R> v = in[i];
+    out[i] = mkc<R>(v.x * s, v.y * s);
+  }
+}
+
+static int64_t spec_entries(const ffhip_plan* p) {
+  int64_t c = p->kind == 0 ? p->n[0] / 2 + 1 : p->n[0];
+  for (int d = 1; d < p->ndim; ++d) c *= p->n[d];
+  return c * p->batch;
+}
+
+static double plan_total(const ffhip_plan* p) {
+  double t = 1;
+  for (int d = 0; d < p->ndim; ++d) t *= (double)p->n[d];
+  return t;
+}
+
+// ldiv!^T(phi) = W * mul!(phi) / prod(n)  (W = 1 at kx = 0, nx/2, else 2; c2c: W = 1)
+template <typename R>
+static int exec_inverse_adjoint(ffhip_plan* p, const void* in, void* out, hipStream_t st) {
+  FF_TRY(exec_forward<R>(p, in, out, st));
+  const double t = plan_total(p);
+  const int64_t cnt = spec_entries(p);
+  const R edge = (R)(1.0 / t), mid = (R)((p->kind == 0 ? 2.0 : 1.0) / t);
+  hipLaunchKernelGGL((k_rfft_weight<R>), dim3(blocks_for(cnt, 256, 8192)), dim3(256), 0, st,
+                     reinterpret_cast<const cplx<R>*>(out), reinterpret_cast<cplx<R>*>(out), cnt,
+                     p->kind == 0 ? p->n[0] / 2 + 1 : p->n[0], edge, mid);
+  FF_LAUNCHED("k_rfft_weight");
+  return FFHIP_OK;
+}
+
+// mul!^T(Y) = prod(n) * ldiv!(Y / W)
+template <typename R>
+static int exec_forward_adjoint(ffhip_plan* p, const void* in, void* out, hipStream_t st) {
+  const double t = plan_total(p);
+  if (p->kind == 1) return exec_inverse<R>(p, in, out, st, t);
+  const int64_t cnt = spec_entries(p);
+  if (!p->adj) FF_TRY(dev_alloc(&p->adj, (size_t)cnt * sizeof(cplx<R>)));
+  hipLaunchKernelGGL((k_rfft_weight<R>), dim3(blocks_for(cnt, 256, 8192)), dim3(256), 0, st,
+                     reinterpret_cast<const cplx<R>*>(in), reinterpret_cast<cplx<R>*>(p->adj), cnt,
+                     p->n[0] / 2 + 1, R(1), R(0.5));
+  FF_LAUNCHED("k_rfft_weight");
+  return exec_inverse<R>(p, p->adj, out, st, t);
 }
 
 // ----------------------------------------------------------------- plan creation
@@ -581,6 +634,7 @@ static void destroy_plan(ffhip_plan* p) {
   dev_free(p->g0);
   dev_free(p->g1);
   dev_free(p->g2);
+  dev_free(p->adj);
   delete p;
 }
 
@@ -661,6 +715,22 @@ int ffhip_plan_inverse(ffhip_plan* p, const void* in, void* out, void* stream) {
   FF_CHECK(in != out, FFHIP_ERR_ARG, "in-place transforms are not supported");
   return p->prec == FFHIP_F32 ? exec_inverse<float>(p, in, out, as_stream(stream))
                               : exec_inverse<double>(p, in, out, as_stream(stream));
+}
+
+int ffhip_plan_forward_adjoint(ffhip_plan* p, const void* in, void* out, void* stream) {
+  clear_error();
+  FF_CHECK(p && in && out, FFHIP_ERR_ARG, "null plan or buffer");
+  FF_CHECK(in != out, FFHIP_ERR_ARG, "in-place transforms are not supported");
+  return p->prec == FFHIP_F32 ? exec_forward_adjoint<float>(p, in, out, as_stream(stream))
+                              : exec_forward_adjoint<double>(p, in, out, as_stream(stream));
+}
+
+int ffhip_plan_inverse_adjoint(ffhip_plan* p, const void* in, void* out, void* stream) {
+  clear_error();
+  FF_CHECK(p && in && out, FFHIP_ERR_ARG, "null plan or buffer");
+  FF_CHECK(in != out, FFHIP_ERR_ARG, "in-place transforms are not supported");
+  return p->prec == FFHIP_F32 ? exec_inverse_adjoint<float>(p, in, out, as_stream(stream))
+                              : exec_inverse_adjoint<double>(p, in, out, as_stream(stream));
 }
 
 int ffhip_plan_info(const ffhip_plan* p, int* ndim, int64_t* n, int64_t* batch, int* prec, int* kind) {

@@ -32,5 +32,10 @@ int op_calcN(ffhip_op* op, void* N, const void* sol, hipStream_t st);
 int fused_create(ffhip_op* op);
 void fused_destroy(ffhip_op* op);
 int fused_etdrk4_run(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, hipStream_t st);
+// discrete adjoint (vort2d_adjoint.hip)
+int fused_forward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void* solT, int64_t nsteps,
+                       int64_t ncheck, hipStream_t st);
+int fused_adjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, void* lam0, int64_t nsteps, int64_t ncheck,
+                  hipStream_t st);
 
 }  // namespace ffhip

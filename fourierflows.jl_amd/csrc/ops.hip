@@ -439,6 +439,32 @@ int ffhip_op_rk4_run(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t 
   return rk4_run_generic(op, ts, sol, h, nsteps, as_stream(stream));
 }
 
+int ffhip_op_etdrk4_forward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void* solT, int64_t nsteps,
+                                 int64_t ncheck, void* stream) {
+  clear_error();
+  FF_CHECK(op && ts && sol0 && solT, FFHIP_ERR_ARG, "null argument");
+  FF_CHECK(nsteps >= 1 && ncheck >= 1 && ncheck <= nsteps, FFHIP_ERR_ARG,
+           "need nsteps >= 1 and 1 <= ncheck <= nsteps (got %lld, %lld)", (long long)nsteps, (long long)ncheck);
+  FF_CHECK(same_layout(op, ts->ndim, ts->dims, ts->prec), FFHIP_ERR_ARG,
+           "stepper shape/precision does not match the operator's rfft layout");
+  FF_CHECK(op->fused && !ts->cplx_coef, FFHIP_ERR_UNSUPPORTED,
+           "the adjoint runs on the fused path only (power-of-two 2D vorticity grid, real L)");
+  return fused_forward_ckpt(op, ts, sol0, solT, nsteps, ncheck, as_stream(stream));
+}
+
+int ffhip_op_etdrk4_adjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, void* lam0, int64_t nsteps,
+                            int64_t ncheck, void* stream) {
+  clear_error();
+  FF_CHECK(op && ts && lamT && lam0, FFHIP_ERR_ARG, "null argument");
+  FF_CHECK(nsteps >= 1 && ncheck >= 1 && ncheck <= nsteps, FFHIP_ERR_ARG,
+           "need nsteps >= 1 and 1 <= ncheck <= nsteps (got %lld, %lld)", (long long)nsteps, (long long)ncheck);
+  FF_CHECK(same_layout(op, ts->ndim, ts->dims, ts->prec), FFHIP_ERR_ARG,
+           "stepper shape/precision does not match the operator's rfft layout");
+  FF_CHECK(op->fused && !ts->cplx_coef, FFHIP_ERR_UNSUPPORTED,
+           "the adjoint runs on the fused path only (power-of-two 2D vorticity grid, real L)");
+  return fused_adjoint(op, ts, lamT, lam0, nsteps, ncheck, as_stream(stream));
+}
+
 int ffhip_op_is_fused(const ffhip_op* op, int* fused) {
   clear_error();
   FF_CHECK(op && fused, FFHIP_ERR_ARG, "null argument");

@@ -36,4 +36,5 @@ struct ffhip_plan {
   void* g0 = nullptr;   // global-path ping-pong buffers
   void* g1 = nullptr;
   void* g2 = nullptr;
+  void* adj = nullptr;  // spectral-size buffer of the transposed inverse (allocated on first use)
 };

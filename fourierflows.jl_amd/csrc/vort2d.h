@@ -1,0 +1,133 @@
+// Shared definitions of the fused 2D vorticity kernels (forward and adjoint).
+#pragma once
+
+#include <cstdint>
+
+#include "ops.h"
+#include "prof.h"
+#include "team_fft.h"
+
+namespace ffhip {
+
+#ifndef FF_ROW_PREFETCH_F32
+#define FF_ROW_PREFETCH_F32 0
+#endif
+#ifndef FF_ROW_PREFETCH_F64
+#define FF_ROW_PREFETCH_F64 1
+#endif
+#ifndef FF_STAGE_UNROLL
+#define FF_STAGE_UNROLL 16  // stage-combination loads in flight per thread
+#endif
+
+constexpr int kVCB = 4;  // kx columns per layout block (tile width)
+#ifndef FF_CBK
+#define FF_CBK 2  // kx columns per column-kernel workgroup (a sub-block of the tile)
+#endif
+#ifndef FF_RRK
+#define FF_RRK 2  // y rows per row-kernel workgroup
+#endif
+constexpr int kCBK = FF_CBK;
+constexpr int kRRK = FF_RRK;
+constexpr int kCSUB = kVCB / kCBK;  // column workgroups sharing one tile
+constexpr int kRSUB = 4 / kRRK;     // row workgroups sharing the 4-row, 128 B chunks
+constexpr int kXCD = 8;             // workgroups are dispatched round-robin over 8 XCDs
+
+// Workgroups that share cache lines are placed on the same XCD (same L2):
+// block b runs on XCD b % 8, so sub-block s of group q gets
+// b = (q % 8) + 8 * ((q / 8) * NSUB + s).
+__device__ __forceinline__ void xcd_split(int b, int nsub, int* group, int* sub) {
+  const int x = b % kXCD;
+  const int slot = b / kXCD;
+  *sub = slot % nsub;
+  *group = (slot / nsub) * kXCD + x;
+}
+
+// points per thread of the y (column) and x (row) transforms
+constexpr int vpts_col(int n) { return n >= 256 ? 16 : 8; }
+constexpr int vpts_row(int n) { return n >= 256 ? 16 : 8; }
+
+struct VortGeom {
+  int nx, ny, nkr;
+  int KX, KXp, TX;  // kept kx count, padded to CB, column blocks
+  int ylo, yhi, KY; // aliased rows [ylo, yhi] (0-based) and kept row count
+  double dk, dl;    // 2 pi / Lx, 2 pi / Ly
+  double s2;        // 1 / (nx ny)^2: normalisation of the product of two unnormalised inverses
+};
+
+struct VortState {
+  VortGeom g;
+  void* U = nullptr;   // state (blocked, kept band)
+  void* N1 = nullptr;  // N(u)
+  void* N2 = nullptr;  // N(a), then N(a) + N(b)
+  void* A = nullptr;   // stage state a
+  void* I[3] = {nullptr, nullptr, nullptr};  // row-kernel inputs: zeta^, u^, v^ (y-inverted)
+  void* O[2] = {nullptr, nullptr};           // row-kernel outputs: (u zeta)^, (v zeta)^ (x-forward)
+  void* twx = nullptr;  // exp(-2 pi i k/nx), k < nx
+  void* twy = nullptr;  // exp(-2 pi i k/ny), k < ny
+  // discrete adjoint (allocated on first use)
+  void* UB = nullptr;    // ubar
+  void* N1B = nullptr;   // N1bar
+  void* N2B = nullptr;   // 2 beta lambda, then N2bar
+  void* AB = nullptr;    // abar
+  void* LAM = nullptr;   // adjoint state at the end of the window (in), start (out)
+  void* IA[2] = {nullptr, nullptr};           // adjoint row inputs: y-inverted i kx s / W, i l s / W
+  void* OA[3] = {nullptr, nullptr, nullptr};  // adjoint row outputs (x-forward): u-, v-, zeta-adjoint terms
+  void* CK = nullptr;    // checkpoints (blocked kept band)
+  int64_t ck_cap = 0;    // checkpoints allocated
+  void* FS = nullptr;    // stored stage fields of one recomputed segment
+  int64_t fs_cap = 0;    // steps of stored fields allocated
+  // the checkpointed forward run the checkpoints belong to
+  const void* ck_ts = nullptr;
+  int64_t ck_nsteps = -1, ck_ncheck = 0;
+};
+
+__device__ __forceinline__ int vfreq(int i, int n) { return i < n / 2 ? i : i - n; }
+
+// compact-row index of kept row iy
+__device__ __forceinline__ int v_ky(int iy, const VortGeom& g) { return iy < g.ylo ? iy : iy - (g.yhi - g.ylo + 1); }
+__device__ __forceinline__ int v_iy(int ky, const VortGeom& g) { return ky < g.ylo ? ky : ky + (g.yhi - g.ylo + 1); }
+
+struct VortColArgs {
+  const void* O0;
+  const void* O1;
+  void* I0;
+  void* I1;
+  void* I2;
+  void* U;
+  void* N1;
+  void* N2;
+  void* A;
+  const double* E;    // exp(dt L)
+  const double* Eh;   // exp(dt L / 2)
+  const double* Z;    // zeta
+  const double* Al;   // alpha
+  const double* Be;   // beta
+  const double* Ga;   // Gamma
+  const void* tw;
+  int do_inverse;
+};
+
+struct VortRowArgs {
+  const void* I0;
+  const void* I1;
+  const void* I2;
+  void* O0;
+  void* O1;
+  const void* tw;
+  void* F;  // optional store of this stage's physical fields (u + i v, zeta), null = off
+};
+
+// stored fields of one stage: u + i v (complex, normalised) then zeta (real), each [ny][nx]
+template <typename R> inline size_t vfield_bytes(const VortGeom& g) {
+  return (size_t)g.nx * g.ny * (sizeof(cplx<R>) + sizeof(R));
+}
+
+// host launchers implemented in vort2d_fused.hip
+int vort_row(int prec, const VortGeom& g, const VortRowArgs& a, hipStream_t st);
+int vort_col(int prec, int stage, const VortGeom& g, const VortColArgs& a, hipStream_t st);
+int vort_import(int prec, const VortGeom& g, const void* sol, void* U, hipStream_t st);
+int vort_export(int prec, const VortGeom& g, void* sol, const void* U, const double* E, int64_t nsteps,
+                hipStream_t st);
+int vort_ensure_packed(ffhip_op* op, ffhip_etdrk4* ts, const VortGeom& g, hipStream_t st);
+
+}  // namespace ffhip

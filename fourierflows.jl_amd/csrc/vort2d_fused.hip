@@ -28,100 +28,10 @@
 #include <new>
 #include <vector>
 
-#include "ops.h"
-#include "prof.h"
-#include "team_fft.h"
+#include "vort2d.h"
 
 namespace ffhip {
 
-#ifndef FF_COL_PREFETCH
-#define FF_COL_PREFETCH 0
-#endif
-#ifndef FF_ROW_PREFETCH_F32
-#define FF_ROW_PREFETCH_F32 0
-#endif
-#ifndef FF_ROW_PREFETCH_F64
-#define FF_ROW_PREFETCH_F64 1
-#endif
-#ifndef FF_ROW_ZETA_FIRST
-#define FF_ROW_ZETA_FIRST 0
-#endif
-#ifndef FF_STAGE_UNROLL
-#define FF_STAGE_UNROLL 16  // stage-combination loads in flight per thread
-#endif
-
-constexpr int kVCB = 4;  // kx columns per layout block (tile width)
-#ifndef FF_CBK
-#define FF_CBK 2  // kx columns per column-kernel workgroup (a sub-block of the tile)
-#endif
-#ifndef FF_RRK
-#define FF_RRK 2  // y rows per row-kernel workgroup
-#endif
-constexpr int kCBK = FF_CBK;
-constexpr int kRRK = FF_RRK;
-constexpr int kCSUB = kVCB / kCBK;  // column workgroups sharing one tile
-constexpr int kRSUB = 4 / kRRK;     // row workgroups sharing the 4-row, 128 B chunks
-constexpr int kXCD = 8;             // workgroups are dispatched round-robin over 8 XCDs
-
-// Workgroups that share cache lines are placed on the same XCD (same L2):
-// block b runs on XCD b % 8, so sub-block s of group q gets
-// b = (q % 8) + 8 * ((q / 8) * NSUB + s).
-__device__ __forceinline__ void xcd_split(int b, int nsub, int* group, int* sub) {
-  const int x = b % kXCD;
-  const int slot = b / kXCD;
-  *sub = slot % nsub;
-  *group = (slot / nsub) * kXCD + x;
-}
-
-// points per thread of the y (column) and x (row) transforms
-constexpr int vpts_col(int n) { return n >= 256 ? 16 : 8; }
-constexpr int vpts_row(int n) { return n >= 256 ? 16 : 8; }
-
-struct VortGeom {
-  int nx, ny, nkr;
-  int KX, KXp, TX;  // kept kx count, padded to CB, column blocks
-  int ylo, yhi, KY; // aliased rows [ylo, yhi] (0-based) and kept row count
-  double dk, dl;    // 2 pi / Lx, 2 pi / Ly
-  double s2;        // 1 / (nx ny)^2: normalisation of the product of two unnormalised inverses
-};
-
-struct VortState {
-  VortGeom g;
-  void* U = nullptr;   // state (blocked, kept band)
-  void* N1 = nullptr;  // N(u)
-  void* N2 = nullptr;  // N(a), then N(a) + N(b)
-  void* A = nullptr;   // stage state a
-  void* I[3] = {nullptr, nullptr, nullptr};  // row-kernel inputs: zeta^, u^, v^ (y-inverted)
-  void* O[2] = {nullptr, nullptr};           // row-kernel outputs: (u zeta)^, (v zeta)^ (x-forward)
-  void* twx = nullptr;  // exp(-2 pi i k/nx), k < nx
-  void* twy = nullptr;  // exp(-2 pi i k/ny), k < ny
-};
-
-__device__ __forceinline__ int vfreq(int i, int n) { return i < n / 2 ? i : i - n; }
-
-// compact-row index of kept row iy
-__device__ __forceinline__ int v_ky(int iy, const VortGeom& g) { return iy < g.ylo ? iy : iy - (g.yhi - g.ylo + 1); }
-__device__ __forceinline__ int v_iy(int ky, const VortGeom& g) { return ky < g.ylo ? ky : ky + (g.yhi - g.ylo + 1); }
-
-struct VortColArgs {
-  const void* O0;
-  const void* O1;
-  void* I0;
-  void* I1;
-  void* I2;
-  void* U;
-  void* N1;
-  void* N2;
-  void* A;
-  const double* E;    // exp(dt L)
-  const double* Eh;   // exp(dt L / 2)
-  const double* Z;    // zeta
-  const double* Al;   // alpha
-  const double* Be;   // beta
-  const double* Ga;   // Gamma
-  const void* tw;
-  int do_inverse;
-};
 
 // STAGE 0: y-inverse of U only.  STAGE 1..4: ETDRK4 stages.
 template <typename R, int NY, int P, int STAGE>
@@ -151,17 +61,9 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vcol(VortGeom g, VortColArgs 
     const cplx<R>* O1 = reinterpret_cast<const cplx<R>*>(a.O1);
 #pragma unroll
     for (int p = 0; p < P; ++p) nn[p] = O0[ibase + (int64_t)(tid + p * T) * CB];
-#if FF_COL_PREFETCH
-    // issue the second array's loads before the first transform so their
-    // latency hides behind its arithmetic
-#pragma unroll
-    for (int p = 0; p < P; ++p) vb[p] = O1[ibase + (int64_t)(tid + p * T) * CB];
-    team_fft<NY, P, -1, R>(nn, lds, tid, tw);
-#else
     team_fft<NY, P, -1, R>(nn, lds, tid, tw);
 #pragma unroll
     for (int p = 0; p < P; ++p) vb[p] = O1[ibase + (int64_t)(tid + p * T) * CB];
-#endif
     team_fft<NY, P, -1, R>(vb, lds, tid, tw);
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -260,14 +162,6 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vcol(VortGeom g, VortColArgs 
   }
 }
 
-struct VortRowArgs {
-  const void* I0;
-  const void* I1;
-  const void* I2;
-  void* O0;
-  void* O1;
-  const void* tw;
-};
 
 template <typename R, int NX, int P>
 __global__ void __launch_bounds__(kRRK*(NX / P)) k_vrow(VortGeom g, VortRowArgs a) {
@@ -290,51 +184,6 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_vrow(VortGeom g, VortRowArgs 
   const cplx<R>* Vh = reinterpret_cast<const cplx<R>*>(a.I2);
   const int KX = g.KX;
 
-#if FF_ROW_ZETA_FIRST
-  // zeta first: only its real part (P values) stays live across the u + i v transform
-  R zr[P];
-  {
-    cplx<R> w[P];
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-      const int k = tid + p * T;
-      cplx<R> s = mkc<R>(0, 0);
-      if (k == 0) {
-        s = mkc<R>(Zh[iaddr(0)].x, 0);
-      } else if (k < KX) {
-        s = Zh[iaddr(k)];
-      } else if (k > NX - KX) {
-        s = cconj<R>(Zh[iaddr(NX - k)]);
-      }
-      w[p] = s;
-    }
-    team_fft<NX, P, 1, R>(w, lds, tid, tw);
-    const R s2 = (R)g.s2;
-#pragma unroll
-    for (int p = 0; p < P; ++p) zr[p] = w[p].x * s2;
-  }
-  cplx<R> w[P];
-#pragma unroll
-  for (int p = 0; p < P; ++p) {
-    const int k = tid + p * T;
-    cplx<R> s = mkc<R>(0, 0);
-    if (k == 0) {
-      const cplx<R> U = Uh[iaddr(0)], V = Vh[iaddr(0)];
-      s = mkc<R>(U.x, V.x);
-    } else if (k < KX) {
-      const cplx<R> U = Uh[iaddr(k)], V = Vh[iaddr(k)];
-      s = mkc<R>(U.x - V.y, U.y + V.x);
-    } else if (k > NX - KX) {
-      const int k2 = NX - k;
-      const cplx<R> U = Uh[iaddr(k2)], V = Vh[iaddr(k2)];
-      s = mkc<R>(U.x + V.y, V.x - U.y);
-    }
-    w[p] = s;
-  }
-  team_fft<NX, P, 1, R>(w, lds, tid, tw);
-#pragma unroll
-  for (int p = 0; p < P; ++p) w[p] = mkc<R>(w[p].x * zr[p], w[p].y * zr[p]);
-#else
   // u + i v: Hermitian-extended half spectra packed into one complex inverse
   cplx<R> z[P];
 #pragma unroll
@@ -378,12 +227,21 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_vrow(VortGeom g, VortRowArgs 
 
   // products u zeta + i v zeta with the 1/(nx ny)^2 normalisation
   const R s2 = (R)g.s2;
+  if (a.F) {  // keep u + i v and zeta (normalised) for the adjoint sweep
+    const R s1 = (R)(1.0 / ((double)g.nx * (double)g.ny));
+    cplx<R>* Fz = reinterpret_cast<cplx<R>*>(a.F) + (int64_t)y * NX;
+    R* Fr = reinterpret_cast<R*>(reinterpret_cast<cplx<R>*>(a.F) + (int64_t)g.nx * g.ny) + (int64_t)y * NX;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      Fz[tid + p * T] = mkc<R>(z[p].x * s1, z[p].y * s1);
+      Fr[tid + p * T] = w[p].x * s1;
+    }
+  }
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     const R zz = w[p].x * s2;
     w[p] = mkc<R>(z[p].x * zz, z[p].y * zz);
   }
-#endif
   team_fft<NX, P, -1, R>(w, lds, tid, tw);
 
   // split W into the half spectra of its real and imaginary parts
@@ -583,6 +441,9 @@ static void vstate_free(VortState* s) {
   for (auto& p : s->O) dev_free(p);
   dev_free(s->twx);
   dev_free(s->twy);
+  void* adj[] = {s->UB, s->N1B, s->N2B, s->AB, s->LAM, s->IA[0], s->IA[1], s->OA[0], s->OA[1], s->OA[2],
+                 s->CK, s->FS};
+  for (void* p : adj) dev_free(p);
   delete s;
 }
 
@@ -647,7 +508,7 @@ static uint64_t geom_key(const VortGeom& g) {
   return k;
 }
 
-static int ensure_packed(ffhip_op* op, ffhip_etdrk4* ts, const VortGeom& g, hipStream_t st) {
+int vort_ensure_packed(ffhip_op* op, ffhip_etdrk4* ts, const VortGeom& g, hipStream_t st) {
   const uint64_t key = geom_key(g);
   if (ts->packed && ts->packed_owner == op && ts->packed_key == key) return FFHIP_OK;
   dev_free(ts->packed);
@@ -669,7 +530,7 @@ template <typename R>
 static int vrun(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, hipStream_t st) {
   VortState* s = reinterpret_cast<VortState*>(op->fused);
   const VortGeom& g = s->g;
-  FF_TRY(ensure_packed(op, ts, g, st));
+  FF_TRY(vort_ensure_packed(op, ts, g, st));
   const size_t n = (size_t)g.TX * g.KY * kVCB;
   const double* pc = reinterpret_cast<const double*>(ts->packed);
   hipLaunchKernelGGL((k_vimport<R>), dim3(blocks_for((int64_t)n, 256, 4096)), dim3(256), 0, st,
@@ -700,6 +561,7 @@ static int vrun(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, hipSt
   ra.O0 = s->O[0];
   ra.O1 = s->O[1];
   ra.tw = s->twx;
+  ra.F = nullptr;
   FF_TRY(dispatch_stage<R>(0, g, ca, st));
   for (int64_t k = 0; k < nsteps; ++k) {
     for (int stage = 1; stage <= 4; ++stage) {
@@ -712,6 +574,39 @@ static int vrun(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, hipSt
   hipLaunchKernelGGL((k_vexport<R>), dim3(blocks_for(total, 256, 4096)), dim3(256), 0, st,
                      reinterpret_cast<cplx<R>*>(sol), reinterpret_cast<const cplx<R>*>(s->U),
                      reinterpret_cast<const double*>(ts->coef[4]), nsteps, g);
+  FF_LAUNCHED("k_vexport");
+  return FFHIP_OK;
+}
+
+int vort_row(int prec, const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
+  return prec == FFHIP_F32 ? dispatch_row<float>(g, a, st) : dispatch_row<double>(g, a, st);
+}
+
+int vort_col(int prec, int stage, const VortGeom& g, const VortColArgs& a, hipStream_t st) {
+  return prec == FFHIP_F32 ? dispatch_stage<float>(stage, g, a, st) : dispatch_stage<double>(stage, g, a, st);
+}
+
+int vort_import(int prec, const VortGeom& g, const void* sol, void* U, hipStream_t st) {
+  const int64_t n = (int64_t)g.TX * g.KY * kVCB;
+  if (prec == FFHIP_F32)
+    hipLaunchKernelGGL((k_vimport<float>), dim3(blocks_for(n, 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<const float2*>(sol), reinterpret_cast<float2*>(U), g);
+  else
+    hipLaunchKernelGGL((k_vimport<double>), dim3(blocks_for(n, 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<const double2*>(sol), reinterpret_cast<double2*>(U), g);
+  FF_LAUNCHED("k_vimport");
+  return FFHIP_OK;
+}
+
+int vort_export(int prec, const VortGeom& g, void* sol, const void* U, const double* E, int64_t nsteps,
+                hipStream_t st) {
+  const int64_t total = (int64_t)g.ny * g.nkr;
+  if (prec == FFHIP_F32)
+    hipLaunchKernelGGL((k_vexport<float>), dim3(blocks_for(total, 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<float2*>(sol), reinterpret_cast<const float2*>(U), E, nsteps, g);
+  else
+    hipLaunchKernelGGL((k_vexport<double>), dim3(blocks_for(total, 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<double2*>(sol), reinterpret_cast<const double2*>(U), E, nsteps, g);
   FF_LAUNCHED("k_vexport");
   return FFHIP_OK;
 }

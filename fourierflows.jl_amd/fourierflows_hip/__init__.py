@@ -12,12 +12,14 @@ include/ffhip.h) that keeps the reference's API names:
 * ``parsevalsum2``, ``parsevalsum``               (src/utils.jl)
 * built-in operators ``VorticityOperator`` / ``TracerOperator`` and the
   ``Diffusion`` testbed                              (src/diffusion.jl)
+* ``adjoint``: differentiable ``ldiv``/``mul``/``dealias`` and the discrete
+  ETDRK4 adjoint (no counterpart in the reference)
 
 Device arrays are torch tensors on the ROCm device (``HIP()`` is the device
 tag, the analogue of ``GPU()``).
 """
 
-from . import _lib
+from . import _lib, adjoint
 from ._lib import DomainError, FFHIPError, NonFiniteError
 from .core import CPU, GPU, HIP, cxtype, device_array, fltype, init, zeros
 from .domains import (OneDGrid, Plan, ThreeDGrid, TwoDGrid, dealias, getaliasedwavenumbers, gridpoints, ldiv,

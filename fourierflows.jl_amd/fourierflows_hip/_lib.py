@@ -68,6 +68,8 @@ SIGNATURES = {
     "ffhip_plan_destroy": [_vp],
     "ffhip_plan_forward": [_vp, _vp, _vp, _vp],
     "ffhip_plan_inverse": [_vp, _vp, _vp, _vp],
+    "ffhip_plan_forward_adjoint": [_vp, _vp, _vp, _vp],
+    "ffhip_plan_inverse_adjoint": [_vp, _vp, _vp, _vp],
     "ffhip_plan_info": [_vp, C.POINTER(_int), _i64p, _i64p, C.POINTER(_int), C.POINTER(_int)],
     "ffhip_aliased_wavenumbers": [_i64, _i64, _dbl, _i64p, _i64p, _i64p, _i64p],
     "ffhip_dealias": [_vp, _int, _int, _i64p, _i64, _i64p, _i64p, _vp],
@@ -94,6 +96,8 @@ SIGNATURES = {
     "ffhip_op_rk4_run": [_vp, _vp, _vp, _dbl, _i64, _vp],
     "ffhip_op_diagnostics": [_vp, _vp, _dp, _vp],
     "ffhip_op_is_fused": [_vp, C.POINTER(_int)],
+    "ffhip_op_etdrk4_forward_ckpt": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
+    "ffhip_op_etdrk4_adjoint": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
     "ffhip_profile_start": [],
     "ffhip_profile_stop": [C.c_char_p, C.c_size_t],
 }

@@ -49,6 +49,16 @@ class Plan:
         _lib.call("ffhip_plan_inverse", self._h, ptr(inp), ptr(out), stream_ptr())
         return out
 
+    def forward_adjoint(self, out, inp):
+        """``mul!^T``: spectral ``inp`` -> physical ``out`` = prod(n) ldiv!(inp / W)."""
+        _lib.call("ffhip_plan_forward_adjoint", self._h, ptr(inp), ptr(out), stream_ptr())
+        return out
+
+    def inverse_adjoint(self, out, inp):
+        """``ldiv!^T``: physical ``inp`` -> spectral ``out`` = W mul!(inp) / prod(n)."""
+        _lib.call("ffhip_plan_inverse_adjoint", self._h, ptr(inp), ptr(out), stream_ptr())
+        return out
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value and _lib._lib is not None:

@@ -84,6 +84,15 @@ int ffhip_plan_forward(ffhip_plan* plan, const void* in, void* out, void* stream
  * (src/diffusion.jl:158-159) are unaffected.  rfft: the imaginary parts of
  * the x-DC and x-Nyquist entries are ignored after the y/z passes, as FFTW. */
 int ffhip_plan_inverse(ffhip_plan* plan, const void* in, void* out, void* stream);
+/* Transposes of the two transforms for the real inner products sum(a*b)
+ * (physical) and Re sum conj(a) b (spectral, stored entries); no counterpart
+ * in the reference, they carry gradients through mul!/ldiv!.
+ * forward_adjoint:  mul!^T(Y) = prod(n) ldiv!(Y / W)      (spectral in, physical out)
+ * inverse_adjoint:  ldiv!^T(phi) = W mul!(phi) / prod(n)  (physical in, spectral out)
+ * W = 1 on the kx = 0 and kx = nx/2 entries of an rfft layout and 2
+ * elsewhere; W = 1 for complex plans. */
+int ffhip_plan_forward_adjoint(ffhip_plan* plan, const void* in, void* out, void* stream);
+int ffhip_plan_inverse_adjoint(ffhip_plan* plan, const void* in, void* out, void* stream);
 /* plan query: ndim, sizes, batch, precision, kind (0 = rfft, 1 = fft). */
 int ffhip_plan_info(const ffhip_plan* plan, int* ndim, int64_t* n, int64_t* batch, int* prec, int* kind);
 
@@ -186,6 +195,22 @@ int ffhip_op_rk4_run(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t 
 /* Diagnostics on the rfft state: out[0] = energy, out[1] = enstrophy
  * (vorticity: 1/2<|u|^2>, 1/2<zeta^2>; tracer: out[0] = 1/2<c^2>, out[1] = <c>). */
 int ffhip_op_diagnostics(ffhip_op* op, const void* sol, double* out, void* stream);
+
+/* Discrete adjoint of nsteps fused ETDRK4 steps.  No counterpart in the
+ * reference (it has no adjoint); for gradient-based estimation of initial
+ * conditions, on the fused path only (power-of-two 2D vorticity, real L).
+ * forward_ckpt: solT = ETDRK4^nsteps(sol0) exactly as ffhip_op_etdrk4_run
+ *   computes it (solT may equal sol0), keeping ncheck uniformly spaced
+ *   checkpoints of the kept band on the device (segment ceil(nsteps/ncheck)).
+ * adjoint: lam0 = (d solT / d sol0)^T lamT, with the transpose taken for the
+ *   real inner product <a, b> = Re sum conj(a) b over the stored rfft entries.
+ *   Recomputes each segment from its checkpoint, keeping the stage fields
+ *   (nx*ny*3 reals per stage) of one segment.  Needs the forward_ckpt call of
+ *   the same op, ts, nsteps and ncheck just before it; lam0 may equal lamT. */
+int ffhip_op_etdrk4_forward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void* solT, int64_t nsteps,
+                                 int64_t ncheck, void* stream);
+int ffhip_op_etdrk4_adjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, void* lam0, int64_t nsteps,
+                            int64_t ncheck, void* stream);
 
 /* Is the fused path active for this operator (power-of-two 2D vorticity grid)? */
 int ffhip_op_is_fused(const ffhip_op* op, int* fused);

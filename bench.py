@@ -75,6 +75,35 @@ def kernel_profile(ff, prob, steps):
     return out
 
 
+def adjoint_timing(ff, torch, n=512, nu=5e-4, dt=0.002, nsteps=500, ncheck=23):
+    """Adjoint-gradient cost against one forward solve (FP32, 512^2, 500 steps,
+    23 checkpoints): J = 1/2 ||zeta(T) - zeta_obs||^2 and dJ/dzeta0."""
+    import numpy as np
+
+    prob = ff.TwoDVorticityProblem(nx=n, nu=nu, dt=dt, T=np.float32)
+    g = prob.grid
+    z0 = torch.zeros(g.ny, g.nx, dtype=torch.float32, device="cuda")
+    ff.ldiv(z0, g.rfftplan, ff.vorticity_ic(g, seed=1))
+    prob.sol.copy_(ff.vorticity_ic(g, seed=2))
+    ff.stepforward(prob, nsteps)
+    zobs = torch.zeros_like(z0)
+    ff.ldiv(zobs, g.rfftplan, prob.sol)
+    ff.adjoint.misfit_gradient(prob, z0, zobs, nsteps, ncheck)  # warm-up (allocates checkpoints)
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    prob.sol.copy_(ff.vorticity_ic(g, seed=1))
+    ff.stepforward(prob, nsteps)
+    e[1].record()
+    e[2].record()
+    ff.adjoint.misfit_gradient(prob, z0, zobs, nsteps, ncheck)
+    e[3].record()
+    torch.cuda.synchronize()
+    tf, tg = e[0].elapsed_time(e[1]), e[2].elapsed_time(e[3])
+    return {"workload": f"{n}^2 FP32, {nsteps} ETDRK4 steps, {ncheck} checkpoints", "forward_ms": round(tf, 3),
+            "gradient_ms": round(tg, 3), "gradient_over_forward": round(tg / tf, 3)}
+
+
 def cpu_baseline(seconds=12.0):
     """FP64 oracle (NumPy + scipy.fft, reference operation order) on host cores."""
     import numpy as np
@@ -181,6 +210,9 @@ def main():
                 "avg_us": round(dom["avg_us"], 3), "bytes_per_launch": dom["bytes_per_launch"]}
     step_bytes = sum(v["bytes_per_launch"] * v["count"] for v in prof32.values()) / a.profile_steps
 
+    adj = adjoint_timing(ff, torch)
+    print(f"# adjoint: {adj}", file=sys.stderr, flush=True)
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(a.cpu_seconds)
@@ -195,6 +227,7 @@ def main():
         "fp64_steps_per_s": round(sps64, 2), "fp32_over_fp64": round(sps32 / sps64, 3),
         "fp32_rel_l2_vs_fp64": {"steps": nacc, "field": err, "energy": abs(e32 - e64) / abs(e64),
                                 "enstrophy": abs(z32 - z64) / abs(z64)},
+        "adjoint": adj,
         "step_algorithmic_GB": round(step_bytes / 1e9, 4),
         "step_GBps": round(step_bytes / (t32 / a.steps) / 1e9, 1),
         "roofline": roofline,

@@ -679,6 +679,11 @@ static int create_plan(ffhip_plan** out, int kind, int ndim, const int64_t* n, i
   return FFHIP_OK;
 }
 
+int plan_inverse_scaled(ffhip_plan* p, const void* in, void* out, double extra, hipStream_t st) {
+  return p->prec == FFHIP_F32 ? exec_inverse<float>(p, in, out, st, extra)
+                              : exec_inverse<double>(p, in, out, st, extra);
+}
+
 }  // namespace ffhip
 
 using namespace ffhip;

@@ -152,6 +152,107 @@ __global__ void k_tracer_assemble(cplx<R>* __restrict__ N, const cplx<R>* __rest
   }
 }
 
+// ----------------------------------------------------------------- tracer adjoint
+// The tracer N is linear in c (steady velocity), so the adjoint of an RK4
+// step needs no forward states.  For the real inner product on the stored
+// rfft entries (W = 1 at kx = 0, nx/2, else 2; Nt = nx ny nz):
+//   N^T(x) = P ldiv!^T( u mul!^T(i kr P x) + v mul!^T(i l P x) + w mul!^T(i m P x) )
+//   mul!^T(Y) = Nt ldiv!(Y / W),  ldiv!^T(phi) = W mul!(phi) / Nt.
+// The 1/W is folded into the prep pass, W/Nt into the finish pass.
+template <typename R>
+__global__ void k_tracer_adj_prep(const cplx<R>* __restrict__ x, cplx<R>* __restrict__ s0, cplx<R>* __restrict__ s1,
+                                  cplx<R>* __restrict__ s2, int64_t nkr, int64_t ny, int64_t nz, double dk, double dl,
+                                  double dm, Kept kept) {
+  const int64_t total = nkr * ny * nz;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ix = idx % nkr;
+    const int64_t r = idx / nkr;
+    const int64_t iy = r % ny, iz = r / ny;
+    if (!kept(ix, iy, iz)) {
+      s0[idx] = s1[idx] = s2[idx] = mkc<R>(0, 0);
+      continue;
+    }
+    const double iw = (ix == 0 || ix == nkr - 1) ? 1.0 : 0.5;
+    const R kr = (R)(dk * (double)ix * iw);
+    const R l = (R)(dl * (double)freq_index(iy, ny) * iw);
+    const R m = (R)(dm * (double)freq_index(iz, nz) * iw);
+    const cplx<R> v = x[idx];
+    s0[idx] = mkc<R>(-kr * v.y, kr * v.x);  // i kr x / W
+    s1[idx] = mkc<R>(-l * v.y, l * v.x);
+    s2[idx] = mkc<R>(-m * v.y, m * v.x);
+  }
+}
+
+template <typename R>
+__global__ void k_tracer_adj_products(const R* __restrict__ p0, const R* __restrict__ p1, const R* __restrict__ p2,
+                                      R* __restrict__ c, int64_t nx, int64_t ny, int64_t nz,
+                                      const double* __restrict__ tab, double A, double B, double C) {
+  const int64_t total = nx * ny * nz;
+  const double* sx = tab;
+  const double* cx = tab + nx;
+  const double* sy = tab + 2 * nx;
+  const double* cy = sy + ny;
+  const double* sz = tab + 2 * nx + 2 * ny;
+  const double* cz = sz + nz;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ix = idx % nx;
+    const int64_t r = idx / nx;
+    const int64_t iy = r % ny, iz = r / ny;
+    const R u = (R)(A * sz[iz] + C * cy[iy]);
+    const R v = (R)(B * sx[ix] + A * cz[iz]);
+    const R w = (R)(C * sy[iy] + B * cx[ix]);
+    c[idx] = u * p0[idx] + v * p1[idx] + w * p2[idx];
+  }
+}
+
+// out = P (W / Nt) S + L x   (L real, FP64)
+template <typename R>
+__global__ void k_tracer_adj_finish(cplx<R>* __restrict__ out, const cplx<R>* __restrict__ S,
+                                    const cplx<R>* __restrict__ x, const double* __restrict__ L, int64_t nkr,
+                                    int64_t ny, int64_t nz, double inv_nt, Kept kept) {
+  const int64_t total = nkr * ny * nz;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ix = idx % nkr;
+    const int64_t r = idx / nkr;
+    const int64_t iy = r % ny, iz = r / ny;
+    const cplx<R> v = x[idx];
+    const double l = L[idx];
+    double ox = l * (double)v.x, oy = l * (double)v.y;
+    if (kept(ix, iy, iz)) {
+      const double w = ((ix == 0 || ix == nkr - 1) ? 1.0 : 2.0) * inv_nt;
+      const cplx<R> s = S[idx];
+      ox += w * (double)s.x;
+      oy += w * (double)s.y;
+    }
+    out[idx] = mkc<R>((R)ox, (R)oy);
+  }
+}
+
+// mode 0: kb = cl lam | 1: ub = lam + o, kb = cl lam + cu o | 2: ub += o, kb = cl lam + cu o | 3: lam = ub + o
+template <typename R>
+__global__ void k_rk4_adj_combo(cplx<R>* __restrict__ ub, cplx<R>* __restrict__ kb, cplx<R>* __restrict__ lam,
+                                const cplx<R>* __restrict__ o, int mode, double cl, double cu, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const cplx<R> l = lam[i];
+    if (mode == 0) {
+      kb[i] = mkc<R>((R)(cl * (double)l.x), (R)(cl * (double)l.y));
+      continue;
+    }
+    const cplx<R> v = o[i];
+    if (mode == 3) {
+      const cplx<R> u = ub[i];
+      lam[i] = mkc<R>(u.x + v.x, u.y + v.y);
+      continue;
+    }
+    const cplx<R> base = mode == 1 ? l : ub[i];
+    ub[i] = mkc<R>(base.x + v.x, base.y + v.y);
+    kb[i] = mkc<R>((R)(cl * (double)l.x + cu * (double)v.x), (R)(cl * (double)l.y + cu * (double)v.y));
+  }
+}
+
 // ----------------------------------------------------------------- diagnostics
 // out: [sum w |u|^2, sum w |u|^2 invK^2] for vorticity; [sum w |u|^2, Re u(0)] for tracer
 template <typename R>
@@ -297,6 +398,72 @@ static int rk4_run_generic(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, in
     FF_TRY(op_calcN(op, R4, s1, st));
     FF_TRY(rk4_addlinear(ts, R4, s1, st));
     FF_TRY(rk4_update(ts, sol, R1, R2, R3, R4, dt, st));
+  }
+  return FFHIP_OK;
+}
+
+// out = (N + L)^T x for the tracer (out, x distinct from the operator's scratch)
+template <typename R>
+static int tracer_rhs_adjoint(ffhip_op* op, const ffhip_rk4* ts, void* out, const void* x, hipStream_t st) {
+  const Kept kept = kept_of(op);
+  const int64_t nkr = op->nkr, nx = op->n[0], ny = op->n[1], nz = op->n[2];
+  const double twopi = 6.283185307179586476925286766559;
+  cplx<R>* s0 = reinterpret_cast<cplx<R>*>(op->spec[0]);
+  cplx<R>* s1 = reinterpret_cast<cplx<R>*>(op->spec[1]);
+  cplx<R>* s2 = reinterpret_cast<cplx<R>*>(op->spec[2]);
+  R* p0 = reinterpret_cast<R*>(op->phys[0]);
+  R* p1 = reinterpret_cast<R*>(op->phys[1]);
+  R* p2 = reinterpret_cast<R*>(op->phys[2]);
+  R* c = reinterpret_cast<R*>(op->phys[3]);
+  const double nt = (double)nx * (double)ny * (double)nz;
+  hipLaunchKernelGGL((k_tracer_adj_prep<R>), dim3(op_blocks(op->spec_count)), dim3(kOpThreads), 0, st,
+                     (const cplx<R>*)x, s0, s1, s2, nkr, ny, nz, twopi / op->box[0], twopi / op->box[1],
+                     twopi / op->box[2], kept);
+  FF_LAUNCHED("k_tracer_adj_prep");
+  FF_TRY(plan_inverse_scaled(op->plan, s0, p0, nt, st));
+  FF_TRY(plan_inverse_scaled(op->plan, s1, p1, nt, st));
+  FF_TRY(plan_inverse_scaled(op->plan, s2, p2, nt, st));
+  hipLaunchKernelGGL((k_tracer_adj_products<R>), dim3(op_blocks(op->phys_count)), dim3(kOpThreads), 0, st,
+                     (const R*)p0, (const R*)p1, (const R*)p2, c, nx, ny, nz, (const double*)op->tables,
+                     op->params[0], op->params[1], op->params[2]);
+  FF_LAUNCHED("k_tracer_adj_products");
+  FF_TRY(ffhip_plan_forward(op->plan, c, s0, st));
+  hipLaunchKernelGGL((k_tracer_adj_finish<R>), dim3(op_blocks(op->spec_count)), dim3(kOpThreads), 0, st,
+                     (cplx<R>*)out, (const cplx<R>*)s0, (const cplx<R>*)x, (const double*)ts->L, nkr, ny, nz,
+                     1.0 / nt, kept);
+  FF_LAUNCHED("k_tracer_adj_finish");
+  return FFHIP_OK;
+}
+
+// lam <- (d sol_T / d sol_0)^T lam over nsteps RK4 steps (reverse order of the
+// forward stages of rk4_run_generic, same step sizes)
+template <typename R>
+static int rk4_adjoint_tracer(ffhip_op* op, ffhip_rk4* ts, void* lam, double dt, int64_t nsteps, hipStream_t st) {
+  FF_TRY(rk4_scratch(ts));
+  cplx<R>* ub = reinterpret_cast<cplx<R>*>(ts->scratch[0]);
+  cplx<R>* kb = reinterpret_cast<cplx<R>*>(ts->scratch[1]);
+  cplx<R>* o = reinterpret_cast<cplx<R>*>(ts->scratch[2]);
+  cplx<R>* l = reinterpret_cast<cplx<R>*>(lam);
+  const double h = dt;
+  const double h2 = op->prec == FFHIP_F32 ? (double)((float)dt / 2.0f) : dt / 2;
+  const double h6 = dt / 6, h3 = 2 * (dt / 6);
+  const int64_t n = ts->count;
+  auto combo = [&](int mode, double cl, double cu) -> int {
+    hipLaunchKernelGGL((k_rk4_adj_combo<R>), dim3(op_blocks(n)), dim3(kOpThreads), 0, st, ub, kb, l,
+                       (const cplx<R>*)o, mode, cl, cu, n);
+    FF_LAUNCHED("k_rk4_adj_combo");
+    return FFHIP_OK;
+  };
+  for (int64_t k = 0; k < nsteps; ++k) {
+    FF_TRY(combo(0, h6, 0));                              // R4bar = h/6 lam
+    FF_TRY((tracer_rhs_adjoint<R>(op, ts, o, kb, st)));   // stage-4 input bar
+    FF_TRY(combo(1, h3, h));                              // ubar = lam + o; R3bar = h/3 lam + h o
+    FF_TRY((tracer_rhs_adjoint<R>(op, ts, o, kb, st)));
+    FF_TRY(combo(2, h3, h2));                             // R2bar = h/3 lam + h/2 o
+    FF_TRY((tracer_rhs_adjoint<R>(op, ts, o, kb, st)));
+    FF_TRY(combo(2, h6, h2));                             // R1bar = h/6 lam + h/2 o
+    FF_TRY((tracer_rhs_adjoint<R>(op, ts, o, kb, st)));
+    FF_TRY(combo(3, 0, 0));                               // lam = ubar + o
   }
   return FFHIP_OK;
 }
@@ -463,6 +630,23 @@ int ffhip_op_etdrk4_adjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, vo
   FF_CHECK(op->fused && !ts->cplx_coef, FFHIP_ERR_UNSUPPORTED,
            "the adjoint runs on the fused path only (power-of-two 2D vorticity grid, real L)");
   return fused_adjoint(op, ts, lamT, lam0, nsteps, ncheck, as_stream(stream));
+}
+
+int ffhip_op_rk4_adjoint(ffhip_op* op, ffhip_rk4* ts, const void* lamT, void* lam0, double dt, int64_t nsteps,
+                         void* stream) {
+  clear_error();
+  FF_CHECK(op && ts && lamT && lam0, FFHIP_ERR_ARG, "null argument");
+  FF_CHECK(nsteps >= 0, FFHIP_ERR_ARG, "nsteps must be >= 0");
+  FF_CHECK(same_layout(op, ts->ndim, ts->dims, ts->prec), FFHIP_ERR_ARG,
+           "stepper shape/precision does not match the operator's rfft layout");
+  FF_CHECK(op->kind == FFHIP_OP_TRACER3D && !ts->cplx_coef, FFHIP_ERR_UNSUPPORTED,
+           "the RK4 adjoint is implemented for the tracer operator with real L");
+  hipStream_t st = as_stream(stream);
+  if (lam0 != lamT)
+    FF_HIP(hipMemcpyAsync(lam0, lamT, (size_t)op->spec_count * cplx_size(op->prec), hipMemcpyDeviceToDevice, st));
+  const double h = op->prec == FFHIP_F32 ? (double)(float)dt : dt;
+  return op->prec == FFHIP_F32 ? rk4_adjoint_tracer<float>(op, ts, lam0, h, nsteps, st)
+                               : rk4_adjoint_tracer<double>(op, ts, lam0, h, nsteps, st);
 }
 
 int ffhip_op_is_fused(const ffhip_op* op, int* fused) {

@@ -38,3 +38,10 @@ struct ffhip_plan {
   void* g2 = nullptr;
   void* adj = nullptr;  // spectral-size buffer of the transposed inverse (allocated on first use)
 };
+
+namespace ffhip {
+// ldiv! with an extra factor: out = extra * ldiv!(in) (extra = prod(n) gives the
+// unnormalised inverse); the input is preserved
+int plan_inverse_scaled(ffhip_plan* p, const void* in, void* out, double extra, hipStream_t st);
+}  // namespace ffhip
+

@@ -24,7 +24,7 @@ from ._lib import DomainError, FFHIPError, NonFiniteError
 from .core import CPU, GPU, HIP, cxtype, device_array, fltype, init, zeros
 from .domains import (OneDGrid, Plan, ThreeDGrid, TwoDGrid, dealias, getaliasedwavenumbers, gridpoints, ldiv,
                       mul)
-from .ics import splitmix64, uniform01, vorticity_ic
+from .ics import splitmix64, tracer_ic, uniform01, vorticity_ic
 from .physics import (Diffusion, TracerOperator, TracerProblem, TwoDVorticityProblem, VorticityOperator,
                       vorticity_linear)
 from .timesteppers import (Clock, EmptyParams, EmptyVars, Equation, ETDRK4TimeStepper, Problem, RK4TimeStepper,

@@ -98,6 +98,7 @@ SIGNATURES = {
     "ffhip_op_is_fused": [_vp, C.POINTER(_int)],
     "ffhip_op_etdrk4_forward_ckpt": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
     "ffhip_op_etdrk4_adjoint": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
+    "ffhip_op_rk4_adjoint": [_vp, _vp, _vp, _vp, _dbl, _i64, _vp],
     "ffhip_profile_start": [],
     "ffhip_profile_stop": [C.c_char_p, C.c_size_t],
 }

@@ -9,7 +9,10 @@ Functions whose backward passes call libffhip's hand-written transposes:
 * ``dealias(grid, X)``: ``dealias!`` on a copy (self-adjoint);
 * ``etdrk4_solve(prob, sol0, nsteps, ncheck)``: ``nsteps`` fused ETDRK4
   steps of a built-in 2D vorticity problem, whose pullback is the discrete
-  adjoint with ``ncheck`` checkpoints and per-segment recompute.
+  adjoint with ``ncheck`` checkpoints and per-segment recompute;
+* ``rk4_solve(prob, sol0, nsteps)``: RK4 steps of the built-in tracer
+  problem, whose pullback is the discrete RK4 adjoint (the equation is
+  linear in c, so no checkpoints are needed).
 
 Complex tensors follow torch's convention: the gradient of a real loss with
 respect to ``z`` is ``dL/dRe z + i dL/dIm z``, which is exactly the
@@ -25,7 +28,7 @@ from __future__ import annotations
 from . import _lib
 from .core import ptr, stream_ptr, zeros
 from .domains import dealias as _dealias
-from .timesteppers import ETDRK4TimeStepper
+from .timesteppers import ETDRK4TimeStepper, RK4TimeStepper
 
 
 def _torch():
@@ -103,6 +106,35 @@ class _ETDRK4Solve:
         return None, lam, None, None
 
 
+class _RK4Solve:
+    @staticmethod
+    def forward(ctx, prob, sol0, nsteps):
+        op = _tracer_operator(prob)
+        solT = sol0.detach().clone().contiguous()
+        _lib.call("ffhip_op_rk4_run", op.handle, prob.timestepper.handle, ptr(solT), float(prob.clock.dt),
+                  int(nsteps), stream_ptr())
+        ctx.prob, ctx.nsteps = prob, int(nsteps)
+        return solT
+
+    @staticmethod
+    def backward(ctx, g):
+        prob = ctx.prob
+        op = _tracer_operator(prob)
+        lam = g.detach().clone().contiguous()
+        _lib.call("ffhip_op_rk4_adjoint", op.handle, prob.timestepper.handle, ptr(lam), ptr(lam),
+                  float(prob.clock.dt), ctx.nsteps, stream_ptr())
+        return None, lam, None
+
+
+def _tracer_operator(prob):
+    op = getattr(prob.eqn.calcN, "builtin_operator", None)
+    if op is None or op.kind != _lib.OP_TRACER3D:
+        raise _lib.FFHIPError(_lib.ERR_UNSUPPORTED, "the RK4 adjoint needs the built-in tracer operator")
+    if not isinstance(prob.timestepper, RK4TimeStepper):
+        raise TypeError("the tracer adjoint is implemented for the RK4 stepper")
+    return op
+
+
 def _spec_T(grid):
     import numpy as np
 
@@ -155,6 +187,39 @@ def etdrk4_solve(prob, sol0, nsteps, ncheck=None):
     if ncheck is None:
         ncheck = max(1, int(round(nsteps**0.5)))
     return _function(_ETDRK4Solve).apply(prob, sol0, int(nsteps), int(ncheck))
+
+
+def rk4_solve(prob, sol0, nsteps):
+    """``nsteps`` RK4 steps of a built-in tracer problem from ``sol0`` (not
+    modified), differentiable in ``sol0``.  The tracer equation is linear in
+    c, so the pullback needs no stored states."""
+    return _function(_RK4Solve).apply(prob, sol0, int(nsteps))
+
+
+def tracer_misfit_gradient(prob, c0, c_obs, nsteps):
+    """(J, dJ/dc0) for ``J = 1/2 sum (ldiv(S^n(P mul(c0))) - c_obs)^2`` with
+    RK4 steps of the built-in tracer operator (explicit library calls)."""
+    torch = _torch()
+    grid = prob.grid
+    op = _tracer_operator(prob)
+    ts = prob.timestepper
+    plan = grid.rfftplan
+    s = zeros(grid.device, _spec_T(grid), grid.spec_dims)
+    _plan_apply("ffhip_plan_forward", plan, s, c0.contiguous())
+    _dealias(s, grid)
+    _lib.call("ffhip_op_rk4_run", op.handle, ts.handle, ptr(s), float(prob.clock.dt), int(nsteps), stream_ptr())
+    cT = zeros(grid.device, grid.T, grid.phys_dims)
+    _plan_apply("ffhip_plan_inverse", plan, cT, s)
+    r = cT - c_obs
+    J = 0.5 * float(torch.sum(r.double() ** 2))
+    lam = s
+    _plan_apply("ffhip_plan_inverse_adjoint", plan, lam, r.contiguous())
+    _lib.call("ffhip_op_rk4_adjoint", op.handle, ts.handle, ptr(lam), ptr(lam), float(prob.clock.dt), int(nsteps),
+              stream_ptr())
+    _dealias(lam, grid)
+    grad = zeros(grid.device, grid.T, grid.phys_dims)
+    _plan_apply("ffhip_plan_forward_adjoint", plan, grad, lam)
+    return J, grad
 
 
 def misfit_gradient(prob, zeta0, zeta_obs, nsteps, ncheck):

@@ -65,3 +65,29 @@ def vorticity_ic(grid, seed, k0=6.0, rms=1.0, spectrum="k4exp1"):
     zh.view(-1)[0] = 0
     dealias(zh, grid)
     return zh
+
+
+def tracer_ic(grid, seed, ngauss=16, sigma=0.3):
+    """Sum of ``ngauss`` periodic Gaussians of width ``sigma`` at centres drawn
+    uniformly in the box (SplitMix64, 3 draws per centre: x, y, z), with
+    minimum-image distances.  Returns the physical field on the device in the
+    grid's float type (C order (nz, ny, nx)); built in FP64 as separable
+    products."""
+    import torch
+
+    u = uniform01(seed, 3 * ngauss).reshape(ngauss, 3)
+    dev = f"cuda:{grid.device.index}"
+    axes = []
+    for d in range(3):
+        n, L, x0 = grid.ns[d], grid.Ls[d], grid.x0s[d]
+        axes.append(torch.as_tensor(x0 + (L / n) * np.arange(n), dtype=torch.float64, device=dev))
+    c = torch.zeros(tuple(reversed(grid.ns)), dtype=torch.float64, device=dev)
+    for g in range(ngauss):
+        f = []
+        for d in range(3):
+            L, x0 = grid.Ls[d], grid.x0s[d]
+            cen = x0 + u[g, d] * L
+            dx = torch.remainder(axes[d] - cen + L / 2, L) - L / 2
+            f.append(torch.exp(-(dx**2) / (2 * sigma**2)))
+        c += f[2][:, None, None] * f[1][None, :, None] * f[0][None, None, :]
+    return c.to(torch.float32 if np.dtype(grid.T) == np.float32 else torch.float64)

@@ -212,6 +212,14 @@ int ffhip_op_etdrk4_forward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol
 int ffhip_op_etdrk4_adjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, void* lam0, int64_t nsteps,
                             int64_t ncheck, void* stream);
 
+/* Discrete adjoint of nsteps RK4 steps of the tracer operator (no
+ * counterpart in the reference): lam0 = (d solT / d sol0)^T lamT for the
+ * steps ffhip_op_rk4_run takes with the same ts and dt, same inner product
+ * as above.  The tracer N is linear in c (steady velocity), so no forward
+ * states, checkpoints or recompute are needed.  lam0 may equal lamT. */
+int ffhip_op_rk4_adjoint(ffhip_op* op, ffhip_rk4* ts, const void* lamT, void* lam0, double dt, int64_t nsteps,
+                         void* stream);
+
 /* Is the fused path active for this operator (power-of-two 2D vorticity grid)? */
 int ffhip_op_is_fused(const ffhip_op* op, int* fused);
 

@@ -144,7 +144,7 @@ def test_autograd_matches_explicit(ff):
     zT = ff.adjoint.ldiv(g, ff.adjoint.etdrk4_solve(prob, s, nsteps, 4))
     Ja = 0.5 * torch.sum((zT - zobs) ** 2)
     Ja.backward()
-    assert abs(float(Ja) - J) < 1e-12 * J
+    assert abs(float(Ja.detach()) - J) < 1e-12 * J
     err = float(torch.linalg.vector_norm(x.grad - grad) / torch.linalg.vector_norm(grad))
     assert err < 1e-13, err
 
@@ -185,6 +185,7 @@ def test_taylor_fp64_512_500_steps(ff):
     J, grad = ff.adjoint.misfit_gradient(prob, z0, zobs, nsteps, 23)
     hs = [1e-1, 1e-2, 1e-3, 1e-4]
     for gd, errs in _taylor(ff, prob, z0, zobs, grad, nsteps, 8, hs, seed=7):
+        print(f"taylor fp64: <g,d>={gd:.6e} errs={['%.3e' % e for e in errs]}")
         assert errs[1] < 0.03 * errs[0] + 1e-9 * abs(gd), (gd, errs)
         assert errs[2] < 0.03 * errs[1] + 1e-9 * abs(gd), (gd, errs)
         assert min(errs) < 1e-6 * abs(gd), (gd, errs)
@@ -200,12 +201,16 @@ def test_fp32_adjoint_512_500_steps(ff):
     J32, g32 = ff.adjoint.misfit_gradient(p32, z0, zobs, nsteps, 23)
     p64, z064, zobs64 = _setup(ff, n, nu, dt, nsteps, np.float64)
     J64, g64 = ff.adjoint.misfit_gradient(p64, z064, zobs64, nsteps, 23)
+    print(f"J fp32 {J32!r} fp64 {J64!r}")
     assert abs(J32 - J64) < 1e-4 * J64, (J32, J64)
     err = float(torch.linalg.vector_norm(g32.double() - g64) / torch.linalg.vector_norm(g64))
+    print(f"gradient fp32 vs fp64 rel l2 {err:.3e}")
     assert err < 1e-4, err
-    # FP32 Taylor: FD of the FP32 misfit against the FP32 gradient; at small h
-    # the FP32 roundoff of J dominates, so the bound is on the best h
+    # Taylor test of the FP32 gradient against central differences of the FP64
+    # misfit (same fields: FP32 roundoff of J itself would swamp the FD at
+    # small h): the error falls as h^2 to the FP32 gradient's own error
     hs = [1e-1, 1e-2, 1e-3, 1e-4]
-    for gd, errs in _taylor(ff, p32, z0, zobs, g32, nsteps, 8, hs, seed=8):
-        assert errs[1] < 0.05 * errs[0] + 1e-3 * abs(gd), (gd, errs)
-        assert min(errs) < 2e-3 * abs(gd), (gd, errs)
+    for gd, errs in _taylor(ff, p64, z064, zobs64, g32, nsteps, 8, hs, seed=8):
+        print(f"taylor fp32 grad: <g,d>={gd:.6e} errs={['%.3e' % e for e in errs]}")
+        assert errs[1] < 0.03 * errs[0] + 1e-4 * abs(gd), (gd, errs)
+        assert min(errs) < 1e-4 * abs(gd), (gd, errs)

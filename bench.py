@@ -143,6 +143,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost measurement")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE)")
@@ -210,8 +211,10 @@ def main():
                 "avg_us": round(dom["avg_us"], 3), "bytes_per_launch": dom["bytes_per_launch"]}
     step_bytes = sum(v["bytes_per_launch"] * v["count"] for v in prof32.values()) / a.profile_steps
 
-    adj = adjoint_timing(ff, torch)
-    print(f"# adjoint: {adj}", file=sys.stderr, flush=True)
+    adj = None
+    if not a.no_adjoint:
+        adj = adjoint_timing(ff, torch)
+        print(f"# adjoint: {adj}", file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu_baseline:

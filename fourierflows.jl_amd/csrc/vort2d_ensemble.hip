@@ -1,0 +1,534 @@
+// Batched ETDRK4 ensembles of 2D vorticity problems on mixed power-of-two
+// grids.  No counterpart in the reference (it steps one problem at a time);
+// each member is the same computation as the single-grid fused path.
+//
+// One launch per pass covers every member: the row kernel and each column
+// stage kernel read a work list of (member, first row / first column) items
+// and run the member's pass with the FFT length templated on its grid size.
+// A workgroup of BD threads holds BD/T teams (T = n/P threads per
+// transform), so small grids get many rows/columns per workgroup and large
+// grids fewer; every team of a workgroup works on the same member, which
+// keeps the team barriers uniform.  Items are ordered largest grid first.
+//
+// Steps are captured once into a HIP graph (10 steps = 80 kernel nodes) and
+// replayed.  The tile/radix variant (BD in {128, 256}; P = 8 everywhere or
+// P = 16 from n = 256 up) is either the single-grid default or picked by
+// timing every valid variant on the ensemble itself.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "vort2d_body.h"
+
+namespace ffhip {
+
+struct EnsMember {  // device-resident descriptor of one member
+  VortGeom g;
+  VortColArgs ca;
+  VortRowArgs ra;
+};
+
+constexpr int kEnsChunk = 10;  // steps per captured graph
+
+constexpr int ens_pts(int n, int prule) { return (prule == 1 && n >= 256) ? 16 : 8; }
+constexpr bool ens_ok(int n, int prule, int bd) { return n / ens_pts(n, prule) <= bd; }
+
+template <typename R, int BD, int PRULE, int N, int STAGE>
+__device__ __forceinline__ void ens_col_n(const VortGeom& g, const VortColArgs& a, int first, unsigned char* sm) {
+  constexpr int P = ens_pts(N, PRULE);
+  constexpr int T = N / P;
+  if constexpr (T <= BD) {
+    constexpr int CPB = BD / T;
+    const int t = threadIdx.x;
+    const int cc = t % CPB;
+    const int col = first + cc;
+    cplx<R>* lds = reinterpret_cast<cplx<R>*>(sm) + cc * TeamSched<N, P>::LDS;
+    vcol_body<R, N, P, STAGE>(g, a, col / kVCB, col % kVCB, t / CPB, lds, col / kVCB < g.TX);
+  }
+}
+
+template <typename R, int BD, int PRULE, int N>
+__device__ __forceinline__ void ens_row_n(const VortGeom& g, const VortRowArgs& a, int first, unsigned char* sm) {
+  constexpr int P = ens_pts(N, PRULE);
+  constexpr int T = N / P;
+  if constexpr (T <= BD) {
+    constexpr int RPB = BD / T;
+    const int t = threadIdx.x;
+    const int r = t % RPB;
+    cplx<R>* lds = reinterpret_cast<cplx<R>*>(sm) + r * TeamSched<N, P>::LDS;
+    vrow_body<R, N, P>(g, a, first + r, t / RPB, lds);
+  }
+}
+
+template <typename R, int BD, int PRULE, int STAGE>
+__global__ void __launch_bounds__(BD) k_ens_col(const EnsMember* __restrict__ mem, const int2* __restrict__ items) {
+  extern __shared__ unsigned char vsm[];
+  const int2 it = items[blockIdx.x];
+  const VortGeom g = mem[it.x].g;
+  const VortColArgs a = mem[it.x].ca;
+  switch (g.ny) {
+    case 64: ens_col_n<R, BD, PRULE, 64, STAGE>(g, a, it.y, vsm); break;
+    case 128: ens_col_n<R, BD, PRULE, 128, STAGE>(g, a, it.y, vsm); break;
+    case 256: ens_col_n<R, BD, PRULE, 256, STAGE>(g, a, it.y, vsm); break;
+    case 512: ens_col_n<R, BD, PRULE, 512, STAGE>(g, a, it.y, vsm); break;
+    default: ens_col_n<R, BD, PRULE, 1024, STAGE>(g, a, it.y, vsm); break;
+  }
+}
+
+template <typename R, int BD, int PRULE>
+__global__ void __launch_bounds__(BD) k_ens_row(const EnsMember* __restrict__ mem, const int2* __restrict__ items) {
+  extern __shared__ unsigned char vsm[];
+  const int2 it = items[blockIdx.x];
+  const VortGeom g = mem[it.x].g;
+  const VortRowArgs a = mem[it.x].ra;
+  switch (g.nx) {
+    case 64: ens_row_n<R, BD, PRULE, 64>(g, a, it.y, vsm); break;
+    case 128: ens_row_n<R, BD, PRULE, 128>(g, a, it.y, vsm); break;
+    case 256: ens_row_n<R, BD, PRULE, 256>(g, a, it.y, vsm); break;
+    case 512: ens_row_n<R, BD, PRULE, 512>(g, a, it.y, vsm); break;
+    default: ens_row_n<R, BD, PRULE, 1024>(g, a, it.y, vsm); break;
+  }
+}
+
+}  // namespace ffhip
+
+using namespace ffhip;
+
+struct EnsVariant {
+  int bd = 128, prule = 1;
+};
+
+struct ffhip_ensemble {
+  int prec = FFHIP_F32;
+  int count = 0;
+  double dt = 0;
+  std::vector<VortGeom> geo;
+  std::vector<EnsMember> hmem;
+  std::vector<void*> Efull;  // exp(dt L) per member, FP64, full rfft layout (export of aliased modes)
+  std::map<int, void*> tw;   // twiddles exp(-2 pi i k/n) per length
+  void* pool = nullptr;      // all member buffers
+  void* dmem = nullptr;      // EnsMember[count]
+  int autotune = 0, tuned = 0;
+  EnsVariant var;
+  double tuned_ms = 0;
+  void* items = nullptr;  // int2[ncol + nrow] for var
+  int ncol = 0, nrow = 0;
+  size_t lds_col = 0, lds_row = 0;
+  hipStream_t cap = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int items_gen = 0, graph_gen = -1;  // the graph holds the item-table pointer it was captured with
+  double bytes[6] = {0, 0, 0, 0, 0, 0};  // algorithmic bytes per pass over all members
+};
+
+namespace ffhip {
+
+static bool pow2i(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+static bool variant_ok(const ffhip_ensemble* e, EnsVariant v) {
+  for (const auto& g : e->geo)
+    if (!ens_ok(g.nx, v.prule, v.bd) || !ens_ok(g.ny, v.prule, v.bd)) return false;
+  return true;
+}
+
+// work lists and LDS sizes of a variant
+static int build_items(ffhip_ensemble* e, EnsVariant v) {
+  std::vector<int> order(e->count);
+  for (int m = 0; m < e->count; ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (int64_t)e->geo[a].nx * e->geo[a].ny > (int64_t)e->geo[b].nx * e->geo[b].ny;
+  });
+  std::vector<int2> cols, rows;
+  size_t lc = 0, lr = 0;
+  const size_t cs = cplx_size(e->prec);
+  for (int m : order) {
+    const VortGeom& g = e->geo[m];
+    const int Py = ens_pts(g.ny, v.prule), Px = ens_pts(g.nx, v.prule);
+    const int cpb = v.bd / (g.ny / Py), rpb = v.bd / (g.nx / Px);
+    FF_CHECK(cpb >= 1 && rpb >= 1, FFHIP_ERR_UNSUPPORTED, "variant %d/%d cannot hold a %d x %d member", v.bd,
+             v.prule, g.nx, g.ny);
+    for (int c = 0; c < g.TX * kVCB; c += cpb) cols.push_back(make_int2(m, c));
+    for (int y = 0; y < g.ny; y += rpb) rows.push_back(make_int2(m, y));
+    lc = std::max(lc, (size_t)cpb * (size_t)(g.ny + g.ny / 16) * cs);
+    lr = std::max(lr, (size_t)rpb * (size_t)(g.nx + g.nx / 16) * cs);
+  }
+  FF_CHECK(lc <= (size_t)max_lds_per_block() && lr <= (size_t)max_lds_per_block(), FFHIP_ERR_UNSUPPORTED,
+           "ensemble variant needs more LDS than a workgroup has");
+  dev_free(e->items);
+  e->items = nullptr;
+  std::vector<int2> all(cols);
+  all.insert(all.end(), rows.begin(), rows.end());
+  FF_TRY(dev_alloc(&e->items, all.size() * sizeof(int2)));
+  FF_HIP(hipMemcpy(e->items, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice));
+  e->ncol = (int)cols.size();
+  e->nrow = (int)rows.size();
+  e->lds_col = lc;
+  e->lds_row = lr;
+  e->var = v;
+  e->items_gen += 1;
+  for (int w = 0; w < 6; ++w) e->bytes[w] = 0;
+  for (const auto& g : e->geo) {
+    for (int st = 0; st <= 4; ++st)
+      e->bytes[st] += e->prec == FFHIP_F32 ? col_bytes<float>(g, st, 1) : col_bytes<double>(g, st, 1);
+    e->bytes[5] += e->prec == FFHIP_F32 ? row_bytes<float>(g) : row_bytes<double>(g);
+  }
+  return FFHIP_OK;
+}
+
+template <typename K>
+static int set_lds(K kern, size_t bytes) {
+  FF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes));
+  return FFHIP_OK;
+}
+
+// which: 0..4 column stage, 5 row pass.  attr_only: set the kernel's LDS
+// limit without launching (done outside any capture).
+template <typename R, int BD, int PRULE>
+static int ens_pass(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only) {
+  const EnsMember* mem = reinterpret_cast<const EnsMember*>(e->dmem);
+  const int2* items = reinterpret_cast<const int2*>(e->items);
+  static const char* names[6] = {"ens_col_stage0", "ens_col_stage1", "ens_col_stage2",
+                                 "ens_col_stage3", "ens_col_stage4", "ens_row"};
+  if (which == 5) {
+    auto k = k_ens_row<R, BD, PRULE>;
+    if (attr_only) return set_lds(k, e->lds_row);
+    prof_begin(names[5], e->bytes[5], st);
+    hipLaunchKernelGGL(k, dim3(e->nrow), dim3(BD), e->lds_row, st, mem, items + e->ncol);
+    prof_end(st);
+    FF_LAUNCHED("k_ens_row");
+    return FFHIP_OK;
+  }
+  void (*k)(const EnsMember*, const int2*) = nullptr;
+  switch (which) {
+    case 0: k = k_ens_col<R, BD, PRULE, 0>; break;
+    case 1: k = k_ens_col<R, BD, PRULE, 1>; break;
+    case 2: k = k_ens_col<R, BD, PRULE, 2>; break;
+    case 3: k = k_ens_col<R, BD, PRULE, 3>; break;
+    default: k = k_ens_col<R, BD, PRULE, 4>; break;
+  }
+  if (attr_only) return set_lds(k, e->lds_col);
+  prof_begin(names[which], e->bytes[which], st);
+  hipLaunchKernelGGL(k, dim3(e->ncol), dim3(BD), e->lds_col, st, mem, items);
+  prof_end(st);
+  FF_LAUNCHED("k_ens_col");
+  return FFHIP_OK;
+}
+
+template <typename R>
+static int ens_pass_v(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only) {
+  const EnsVariant v = e->var;
+  if (v.bd == 128)
+    return v.prule ? ens_pass<R, 128, 1>(e, which, st, attr_only) : ens_pass<R, 128, 0>(e, which, st, attr_only);
+  return v.prule ? ens_pass<R, 256, 1>(e, which, st, attr_only) : ens_pass<R, 256, 0>(e, which, st, attr_only);
+}
+
+static int ens_launch(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only = false) {
+  return e->prec == FFHIP_F32 ? ens_pass_v<float>(e, which, st, attr_only)
+                              : ens_pass_v<double>(e, which, st, attr_only);
+}
+
+static int ens_step(ffhip_ensemble* e, hipStream_t st) {
+  for (int stage = 1; stage <= 4; ++stage) {
+    FF_TRY(ens_launch(e, 5, st));
+    FF_TRY(ens_launch(e, stage, st));
+  }
+  return FFHIP_OK;
+}
+
+static int ens_import(ffhip_ensemble* e, void* const* sols, hipStream_t st) {
+  for (int m = 0; m < e->count; ++m) FF_TRY(vort_import(e->prec, e->geo[m], sols[m], e->hmem[m].ca.U, st));
+  return FFHIP_OK;
+}
+
+static void drop_graph(ffhip_ensemble* e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  e->exec = nullptr;
+}
+
+static int capture_graph(ffhip_ensemble* e) {
+  drop_graph(e);
+  if (!e->cap) FF_HIP(hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking));
+  FF_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
+  int rc = FFHIP_OK;
+  for (int k = 0; k < kEnsChunk && rc == FFHIP_OK; ++k) rc = ens_step(e, e->cap);
+  hipGraph_t graph = nullptr;
+  hipError_t ee = hipStreamEndCapture(e->cap, &graph);
+  if (rc != FFHIP_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ee != hipSuccess) return fail(FFHIP_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ee));
+  ee = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ee != hipSuccess) {
+    e->exec = nullptr;
+    return fail(FFHIP_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ee));
+  }
+  e->graph_gen = e->items_gen;
+  return FFHIP_OK;
+}
+
+// LDS limits of every pass of the current variant
+static int set_variant_attrs(ffhip_ensemble* e) {
+  for (int w = 0; w <= 5; ++w) FF_TRY(ens_launch(e, w, nullptr, true));
+  return FFHIP_OK;
+}
+
+// time every valid variant on the ensemble's own buffers (their contents are
+// overwritten by the import that follows) and keep the fastest
+static int autotune(ffhip_ensemble* e, hipStream_t st) {
+  const EnsVariant cands[4] = {{128, 1}, {128, 0}, {256, 1}, {256, 0}};
+  hipEvent_t a, b;
+  FF_HIP(hipEventCreate(&a));
+  FF_HIP(hipEventCreate(&b));
+  double best = 1e300;
+  EnsVariant bestv = e->var;
+  int rc = FFHIP_OK;
+  for (const EnsVariant& v : cands) {
+    if (!variant_ok(e, v)) continue;
+    if (hipStreamSynchronize(st) != hipSuccess) {  // the item table is replaced below
+      rc = fail(FFHIP_ERR_HIP, "autotune: stream synchronisation failed");
+      break;
+    }
+    rc = build_items(e, v);
+    if (rc != FFHIP_OK) break;
+    rc = set_variant_attrs(e);
+    for (int k = 0; k < 2 && rc == FFHIP_OK; ++k) rc = ens_step(e, st);
+    if (rc != FFHIP_OK) break;
+    (void)hipEventRecord(a, st);
+    for (int k = 0; k < 5 && rc == FFHIP_OK; ++k) rc = ens_step(e, st);
+    (void)hipEventRecord(b, st);
+    if (rc != FFHIP_OK) break;
+    if (hipEventSynchronize(b) != hipSuccess) {
+      rc = fail(FFHIP_ERR_HIP, "autotune timing failed");
+      break;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    if (ms / 5 < best) {
+      best = ms / 5;
+      bestv = v;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  FF_TRY(rc);
+  FF_HIP(hipStreamSynchronize(st));
+  FF_TRY(build_items(e, bestv));
+  e->tuned_ms = best;
+  e->tuned = 1;
+  return FFHIP_OK;
+}
+
+static void ens_free(ffhip_ensemble* e) {
+  if (!e) return;
+  drop_graph(e);
+  if (e->cap) (void)hipStreamDestroy(e->cap);
+  dev_free(e->pool);
+  dev_free(e->dmem);
+  dev_free(e->items);
+  for (void* p : e->Efull) dev_free(p);
+  for (auto& kv : e->tw) dev_free(kv.second);
+  delete e;
+}
+
+}  // namespace ffhip
+
+extern "C" {
+
+int ffhip_ensemble_create(ffhip_ensemble** out, int prec, int count, const int64_t* n, const double* box,
+                          double aliased_fraction, const double* const* L, double dt, int autotune_flag) {
+  clear_error();
+  FF_CHECK(out, FFHIP_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  FF_CHECK(valid_prec(prec), FFHIP_ERR_ARG, "bad precision %d", prec);
+  FF_CHECK(count >= 1 && n && box && L, FFHIP_ERR_ARG, "need count >= 1 and non-null n, box, L");
+  FF_CHECK(aliased_fraction > 0 && aliased_fraction < 1, FFHIP_ERR_ARG,
+           "ensembles need 0 < aliased_fraction < 1 (the fused passes keep the dealiased band only)");
+  ffhip_ensemble* e = new (std::nothrow) ffhip_ensemble();
+  FF_CHECK(e, FFHIP_ERR_OOM, "host allocation failed");
+  e->prec = prec;
+  e->count = count;
+  e->dt = prec == FFHIP_F32 ? (double)(float)dt : dt;
+  e->autotune = autotune_flag;
+  int rc = FFHIP_OK;
+  const size_t cs = cplx_size(prec);
+  size_t total = 0;
+  std::vector<size_t> off(count);
+  const double twopi = 6.283185307179586476925286766559;
+  for (int m = 0; m < count && rc == FFHIP_OK; ++m) {
+    const int64_t nx = n[2 * m], ny = n[2 * m + 1];
+    if (!pow2i(nx) || !pow2i(ny) || nx < 64 || ny < 64 || nx > 1024 || ny > 1024) {
+      rc = fail(FFHIP_ERR_UNSUPPORTED, "member %d: ensemble grids must be powers of two in 64..1024 (got %lld x %lld)",
+                m, (long long)nx, (long long)ny);
+      break;
+    }
+    if (!(box[2 * m] > 0 && box[2 * m + 1] > 0)) {
+      rc = fail(FFHIP_ERR_ARG, "member %d: domain lengths must be positive", m);
+      break;
+    }
+    int64_t klo, khi, krlo, krhi, llo, lhi;
+    rc = ffhip_aliased_wavenumbers(nx, nx / 2 + 1, aliased_fraction, nullptr, nullptr, &krlo, &krhi);
+    if (rc == FFHIP_OK) rc = ffhip_aliased_wavenumbers(ny, ny / 2 + 1, aliased_fraction, &llo, &lhi, &klo, &khi);
+    if (rc != FFHIP_OK) break;
+    VortGeom g;
+    g.nx = (int)nx;
+    g.ny = (int)ny;
+    g.nkr = (int)(nx / 2 + 1);
+    g.KX = (int)krlo - 1;
+    g.KXp = (g.KX + kVCB - 1) / kVCB * kVCB;
+    g.TX = g.KXp / kVCB;
+    g.ylo = (int)llo - 1;
+    g.yhi = (int)lhi - 1;
+    g.KY = g.ny - (g.yhi - g.ylo + 1);
+    g.dk = twopi / box[2 * m];
+    g.dl = twopi / box[2 * m + 1];
+    g.s2 = 1.0 / ((double)nx * (double)ny * (double)nx * (double)ny);
+    if (g.KX < 1 || g.KX > g.nx / 2 || g.ylo < 1 || g.yhi < g.ylo || g.ylo > g.ny / 2 || g.yhi < g.ny / 2) {
+      rc = fail(FFHIP_ERR_UNSUPPORTED, "member %d: aliased_fraction leaves no proper kept band", m);
+      break;
+    }
+    e->geo.push_back(g);
+    off[m] = total;
+    const size_t sb = (size_t)g.TX * g.KY * kVCB * cs, ib = (size_t)g.TX * g.ny * kVCB * cs;
+    const size_t cb = (size_t)g.TX * g.KY * kVCB * sizeof(double);
+    total += 4 * sb + 5 * ib + 6 * cb;
+    total = (total + 255) / 256 * 256;
+  }
+  if (rc == FFHIP_OK) rc = dev_alloc(&e->pool, total);
+  e->hmem.resize(e->geo.size());
+  e->Efull.assign(count, nullptr);
+  for (int m = 0; m < count && rc == FFHIP_OK; ++m) {
+    const VortGeom& g = e->geo[m];
+    const size_t sb = (size_t)g.TX * g.KY * kVCB * cs, ib = (size_t)g.TX * g.ny * kVCB * cs;
+    const size_t nc = (size_t)g.TX * g.KY * kVCB;
+    char* p = reinterpret_cast<char*>(e->pool) + off[m];
+    EnsMember& em = e->hmem[m];
+    em.g = g;
+    VortColArgs& ca = em.ca;
+    ca.U = p;
+    ca.N1 = p + sb;
+    ca.N2 = p + 2 * sb;
+    ca.A = p + 3 * sb;
+    char* q = p + 4 * sb;
+    ca.I0 = q;
+    ca.I1 = q + ib;
+    ca.I2 = q + 2 * ib;
+    ca.O0 = q + 3 * ib;
+    ca.O1 = q + 4 * ib;
+    double* pc = reinterpret_cast<double*>(q + 5 * ib);
+    ca.Z = pc;
+    ca.Al = pc + nc;
+    ca.Be = pc + 2 * nc;
+    ca.Ga = pc + 3 * nc;
+    ca.E = pc + 4 * nc;
+    ca.Eh = pc + 5 * nc;
+    ca.do_inverse = 1;
+    for (int len : {g.nx, g.ny}) {
+      if (!e->tw.count(len)) {
+        void* t = nullptr;
+        rc = upload_roots(len, len, prec, &t);
+        if (rc != FFHIP_OK) break;
+        e->tw[len] = t;
+      }
+    }
+    if (rc != FFHIP_OK) break;
+    ca.tw = e->tw[g.ny];
+    VortRowArgs& ra = em.ra;
+    ra.I0 = ca.I0;
+    ra.I1 = ca.I1;
+    ra.I2 = ca.I2;
+    ra.O0 = q + 3 * ib;
+    ra.O1 = q + 4 * ib;
+    ra.tw = e->tw[g.nx];
+    ra.F = nullptr;
+    // coefficients: the single-grid stepper's own construction, then packed
+    ffhip_etdrk4* ts = nullptr;
+    const int64_t dims[2] = {g.nkr, g.ny};
+    if (!L[m]) {
+      rc = fail(FFHIP_ERR_ARG, "member %d: null L", m);
+      break;
+    }
+    rc = ffhip_etdrk4_create(&ts, L[m], nullptr, 2, dims, dt, prec);
+    if (rc != FFHIP_OK) break;
+    const int64_t ncnt = (int64_t)nc;
+    for (int c = 0; c < 6; ++c) {
+      hipLaunchKernelGGL(k_vpack, dim3(blocks_for(ncnt, 256, 4096)), dim3(256), 0, 0,
+                         reinterpret_cast<const double*>(ts->coef[c]), pc + c * nc, g);
+    }
+    // keep exp(dt L) in the full layout for the aliased modes at export
+    rc = dev_alloc(&e->Efull[m], (size_t)g.nkr * g.ny * sizeof(double));
+    if (rc == FFHIP_OK) {
+      hipError_t he = hipMemcpy(e->Efull[m], ts->coef[4], (size_t)g.nkr * g.ny * sizeof(double),
+                                hipMemcpyDeviceToDevice);
+      if (he != hipSuccess) rc = fail(FFHIP_ERR_HIP, "hipMemcpy failed: %s", hipGetErrorString(he));
+    }
+    if (rc == FFHIP_OK) {
+      hipError_t he = hipDeviceSynchronize();
+      if (he != hipSuccess) rc = fail(FFHIP_ERR_HIP, "coefficient packing failed: %s", hipGetErrorString(he));
+    }
+    ffhip_etdrk4_destroy(ts);
+  }
+  if (rc == FFHIP_OK) rc = dev_alloc(&e->dmem, e->hmem.size() * sizeof(EnsMember));
+  if (rc == FFHIP_OK) {
+    hipError_t he = hipMemcpy(e->dmem, e->hmem.data(), e->hmem.size() * sizeof(EnsMember), hipMemcpyHostToDevice);
+    if (he != hipSuccess) rc = fail(FFHIP_ERR_HIP, "hipMemcpy failed: %s", hipGetErrorString(he));
+  }
+  if (rc == FFHIP_OK) {
+    EnsVariant v;  // the single-grid kernels' radix schedule
+    v.bd = 128;
+    v.prule = 1;
+    if (!variant_ok(e, v)) v.bd = 256;
+    rc = build_items(e, v);
+  }
+  if (rc != FFHIP_OK) {
+    ens_free(e);
+    return rc;
+  }
+  *out = e;
+  return FFHIP_OK;
+}
+
+int ffhip_ensemble_destroy(ffhip_ensemble* e) {
+  clear_error();
+  ens_free(e);
+  return FFHIP_OK;
+}
+
+int ffhip_ensemble_run(ffhip_ensemble* e, void* const* sols, int64_t nsteps, void* stream) {
+  clear_error();
+  FF_CHECK(e && sols, FFHIP_ERR_ARG, "null argument");
+  FF_CHECK(nsteps >= 0, FFHIP_ERR_ARG, "nsteps must be >= 0");
+  for (int m = 0; m < e->count; ++m) FF_CHECK(sols[m], FFHIP_ERR_ARG, "member %d: null state", m);
+  if (nsteps == 0) return FFHIP_OK;
+  hipStream_t st = as_stream(stream);
+  if (e->autotune && !e->tuned) FF_TRY(autotune(e, st));
+  FF_TRY(set_variant_attrs(e));
+  FF_TRY(ens_import(e, sols, st));
+  FF_TRY(ens_launch(e, 0, st));
+  int64_t left = nsteps;
+  const bool graphs = !g_prof_on && nsteps >= kEnsChunk;
+  if (graphs) {
+    if (!e->exec || e->graph_gen != e->items_gen) FF_TRY(capture_graph(e));
+    for (; left >= kEnsChunk; left -= kEnsChunk) FF_HIP(hipGraphLaunch(e->exec, st));
+  }
+  for (; left > 0; --left) FF_TRY(ens_step(e, st));
+  for (int m = 0; m < e->count; ++m)
+    FF_TRY(vort_export(e->prec, e->geo[m], sols[m], e->hmem[m].ca.U, reinterpret_cast<const double*>(e->Efull[m]),
+                       nsteps, st));
+  return FFHIP_OK;
+}
+
+int ffhip_ensemble_info(const ffhip_ensemble* e, int* threads, int* prule, double* ms_per_step) {
+  clear_error();
+  FF_CHECK(e, FFHIP_ERR_ARG, "null ensemble");
+  if (threads) *threads = e->var.bd;
+  if (prule) *prule = e->var.prule;
+  if (ms_per_step) *ms_per_step = e->tuned ? e->tuned_ms : 0.0;
+  return FFHIP_OK;
+}
+
+}  // extern "C"

@@ -24,9 +24,9 @@ from ._lib import DomainError, FFHIPError, NonFiniteError
 from .core import CPU, GPU, HIP, cxtype, device_array, fltype, init, zeros
 from .domains import (OneDGrid, Plan, ThreeDGrid, TwoDGrid, dealias, getaliasedwavenumbers, gridpoints, ldiv,
                       mul)
-from .ics import splitmix64, tracer_ic, uniform01, vorticity_ic
-from .physics import (Diffusion, TracerOperator, TracerProblem, TwoDVorticityProblem, VorticityOperator,
-                      vorticity_linear)
+from .ics import ensemble_members, splitmix64, tracer_ic, uniform01, vorticity_ic
+from .physics import (Diffusion, TracerOperator, TracerProblem, TwoDVorticityProblem, VorticityEnsemble,
+                      VorticityOperator, vorticity_linear)
 from .timesteppers import (Clock, EmptyParams, EmptyVars, Equation, ETDRK4TimeStepper, Problem, RK4TimeStepper,
                            TimeStepper, isexplicit, step_until, stepforward)
 from .utils import etdcoeffs, parsevalsum, parsevalsum2

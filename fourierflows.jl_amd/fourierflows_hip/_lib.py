@@ -99,6 +99,10 @@ SIGNATURES = {
     "ffhip_op_etdrk4_forward_ckpt": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
     "ffhip_op_etdrk4_adjoint": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
     "ffhip_op_rk4_adjoint": [_vp, _vp, _vp, _vp, _dbl, _i64, _vp],
+    "ffhip_ensemble_create": [C.POINTER(_vp), _int, _int, _i64p, _dp, _dbl, C.POINTER(_dp), _dbl, _int],
+    "ffhip_ensemble_destroy": [_vp],
+    "ffhip_ensemble_run": [_vp, C.POINTER(_vp), _i64, _vp],
+    "ffhip_ensemble_info": [_vp, C.POINTER(_int), C.POINTER(_int), _dp],
     "ffhip_profile_start": [],
     "ffhip_profile_stop": [C.c_char_p, C.c_size_t],
 }

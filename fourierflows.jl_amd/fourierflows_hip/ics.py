@@ -91,3 +91,23 @@ def tracer_ic(grid, seed, ngauss=16, sigma=0.3):
             f.append(torch.exp(-(dx**2) / (2 * sigma**2)))
         c += f[2][:, None, None] * f[1][None, :, None] * f[0][None, None, :]
     return c.to(torch.float32 if np.dtype(grid.T) == np.float32 else torch.float64)
+
+
+def ensemble_members(master_seed=2026, counts=((128, 32), (256, 24), (512, 8)), nu_range=(1e-4, 1e-2),
+                     k0_range=(4.0, 10.0)):
+    """Per-member (n, seed, nu, k0) of a mixed-grid ensemble: nu log-uniform
+    on ``nu_range``, k0 uniform on ``k0_range``, seeds and draws from one
+    SplitMix64 stream (reproducible in any language)."""
+    total = sum(c for _, c in counts)
+    u = uniform01(master_seed, 3 * total).reshape(total, 3)
+    out = []
+    i = 0
+    lo, hi = np.log(nu_range[0]), np.log(nu_range[1])
+    for n, c in counts:
+        for _ in range(c):
+            seed = int(splitmix64(master_seed + 7919, 1, offset=i)[0] & np.uint64(0x7FFFFFFF))
+            nu = float(np.exp(lo + (hi - lo) * u[i, 0]))
+            k0 = float(k0_range[0] + (k0_range[1] - k0_range[0]) * u[i, 1])
+            out.append((n, seed, nu, k0))
+            i += 1
+    return out

@@ -209,3 +209,65 @@ class Diffusion:
         v.c.copy_(device_array(prob.grid.device)(np.asarray(c, dtype=prob.grid.T)))
         mul(prob.sol, prob.grid.rfftplan, v.c)
         Diffusion.updatevars(prob)
+
+
+# ------------------------------------------------------------------ ensembles
+class VorticityEnsemble:
+    """A batch of independent 2D vorticity problems on mixed power-of-two
+    grids (64..1024), stepped together by libffhip: one kernel launch per
+    pass covers every member and steps replay from a captured HIP graph.
+    No counterpart in the reference, which steps one ``Problem`` at a time.
+
+    ``members``: list of dicts with keys ``nx`` (and optionally ``ny``,
+    ``Lx``, ``Ly``, ``nu``, ``nu_n``, ``n_hyper``, ``mu``); ``dt`` is common.
+    ``sols[m]`` is member m's spectral state (torch tensor, reference
+    layout) and ``grids[m]`` its ``TwoDGrid``.
+    """
+
+    def __init__(self, members, dt, T=np.float32, aliased_fraction=1 / 3, autotune=False, dev=None):
+        dev = HIP() if dev is None else dev
+        self.T = T
+        self.dt = dt
+        self.grids, self.sols, Ls = [], [], []
+        n, box = [], []
+        for m in members:
+            nx = int(m["nx"])
+            ny = int(m.get("ny", nx))
+            Lx = float(m.get("Lx", 2 * np.pi))
+            Ly = float(m.get("Ly", Lx))
+            g = TwoDGrid(dev, nx=nx, Lx=Lx, ny=ny, Ly=Ly, x0=0.0, y0=0.0, T=T, aliased_fraction=aliased_fraction)
+            L = vorticity_linear(g, m.get("nu", 0.0), m.get("nu_n", 0.0), m.get("n_hyper", 1), m.get("mu", 0.0))
+            Ls.append(np.ascontiguousarray(np.broadcast_to(L, (ny, g.nkr)), dtype=np.float64))
+            self.grids.append(g)
+            self.sols.append(zeros(dev, np.result_type(T, np.complex64), (g.nkr, ny)))
+            n += [nx, ny]
+            box += [Lx, Ly]
+        self._Ls = Ls  # keep the host arrays alive for the call
+        Lp = (C.POINTER(C.c_double) * len(Ls))(*[_lib.dptr(L) for L in Ls])
+        h = C.c_void_p()
+        _lib.call("ffhip_ensemble_create", C.byref(h), prec_of(T), len(members), _lib.i64arr(n),
+                  _lib.dptr(np.asarray(box, dtype=np.float64)), float(aliased_fraction), Lp, float(dt),
+                  1 if autotune else 0)
+        self._h = h
+        self.step = 0
+
+    def __len__(self):
+        return len(self.sols)
+
+    def run(self, nsteps):
+        """``nsteps`` ETDRK4 steps of every member (states updated in place)."""
+        ptrs = (C.c_void_p * len(self.sols))(*[ptr(s) for s in self.sols])
+        _lib.call("ffhip_ensemble_run", self._h, ptrs, int(nsteps), stream_ptr())
+        self.step += int(nsteps)
+
+    def info(self):
+        """(threads per workgroup, points-per-thread rule, tuned ms per step or 0)."""
+        t, p, ms = C.c_int(), C.c_int(), C.c_double()
+        _lib.call("ffhip_ensemble_info", self._h, C.byref(t), C.byref(p), C.byref(ms))
+        return t.value, p.value, ms.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.ffhip_ensemble_destroy(h)
+            self._h = None

@@ -52,6 +52,7 @@ typedef struct ffhip_plan ffhip_plan;
 typedef struct ffhip_etdrk4 ffhip_etdrk4;
 typedef struct ffhip_rk4 ffhip_rk4;
 typedef struct ffhip_op ffhip_op;
+typedef struct ffhip_ensemble ffhip_ensemble;
 
 /* ---------------------------------------------------------------- runtime
  * Device selection and memory: the HIPArray analogue of `zeros(dev,T,dims)`
@@ -222,6 +223,29 @@ int ffhip_op_rk4_adjoint(ffhip_op* op, ffhip_rk4* ts, const void* lamT, void* la
 
 /* Is the fused path active for this operator (power-of-two 2D vorticity grid)? */
 int ffhip_op_is_fused(const ffhip_op* op, int* fused);
+
+/* ---------------------------------------------------------------- ensembles
+ * No counterpart in the reference (it steps one Problem at a time): a batch
+ * of independent 2D vorticity problems (the FFHIP_OP_VORTICITY2D physics,
+ * ETDRK4 with a common dt) on power-of-two grids of mixed sizes in
+ * 64..1024, stepped together: one kernel launch per pass covers every
+ * member, and steps are replayed from a captured HIP graph.
+ * n[2m], n[2m+1]: member m's nx, ny; box[2m], box[2m+1]: its Lx, Ly;
+ * L[m]: its linear operator (host FP64, nkr*ny entries, x fastest, real).
+ * autotune != 0: on the first run, time each valid tile/radix variant on
+ * the ensemble and keep the fastest (otherwise the single-grid kernels'
+ * radix schedule is used and each member's result is bitwise the one
+ * ffhip_op_etdrk4_run gives). */
+int ffhip_ensemble_create(ffhip_ensemble** e, int prec, int count, const int64_t* n, const double* box,
+                          double aliased_fraction, const double* const* L, double dt, int autotune);
+int ffhip_ensemble_destroy(ffhip_ensemble* e);
+/* nsteps ETDRK4 steps of every member; sols[m] is member m's rfft state on
+ * the device (nkr x ny, state precision), updated in place. */
+int ffhip_ensemble_run(ffhip_ensemble* e, void* const* sols, int64_t nsteps, void* stream);
+/* the variant in use: threads per workgroup, points-per-thread rule
+ * (0: 8 points everywhere, 1: 16 points from n = 256 up) and, if tuned, its
+ * measured ms per ensemble step. */
+int ffhip_ensemble_info(const ffhip_ensemble* e, int* threads, int* prule, double* ms_per_step);
 
 /* ---------------------------------------------------------------- profiling
  * No counterpart in the reference.  Between start and stop the fused step

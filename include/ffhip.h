@@ -234,8 +234,8 @@ int ffhip_op_is_fused(const ffhip_op* op, int* fused);
  * L[m]: its linear operator (host FP64, nkr*ny entries, x fastest, real).
  * autotune != 0: on the first run, time each valid tile/radix variant on
  * the ensemble and keep the fastest (otherwise the single-grid kernels'
- * radix schedule is used and each member's result is bitwise the one
- * ffhip_op_etdrk4_run gives). */
+ * radix schedule is used).  Each member's result agrees with the one
+ * ffhip_op_etdrk4_run gives to a few ulps. */
 int ffhip_ensemble_create(ffhip_ensemble** e, int prec, int count, const int64_t* n, const double* box,
                           double aliased_fraction, const double* const* L, double dt, int autotune);
 int ffhip_ensemble_destroy(ffhip_ensemble* e);

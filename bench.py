@@ -31,6 +31,9 @@ sys.path.insert(0, os.path.join(ROOT, "fourierflows.jl_amd"))
 
 METRIC = "ETDRK4 steps/s at 2048² FP32 vs FP64 on GPU; FFT HBM GB/s; adjoint/forward time"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vorticity_2048_fp32_hbm_traffic.json")
+ROCPROF_NAMES = {"vort_row": "k_vrow<float, 2048, 16>",
+                 **{f"vort_col_stage{s}": f"k_vcol<float, 2048, 16, {s}>" for s in range(5)}}
 N = 2048
 DT = 1e-3
 MU = 0.01
@@ -102,6 +105,17 @@ def adjoint_timing(ff, torch, n=512, nu=5e-4, dt=0.002, nsteps=500, ncheck=23):
     tf, tg = e[0].elapsed_time(e[1]), e[2].elapsed_time(e[3])
     return {"workload": f"{n}^2 FP32, {nsteps} ETDRK4 steps, {ncheck} checkpoints", "forward_ms": round(tf, 3),
             "gradient_ms": round(tg, 3), "gradient_over_forward": round(tg / tf, 3)}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (scaled by the counters' reading of a 256 MiB copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f).get(ROCPROF_NAMES.get(kernel, ""))
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else round(rec["bytes"])
 
 
 def cpu_baseline(seconds=12.0):
@@ -207,8 +221,8 @@ def main():
     # dominant kernel of the FP32 step: the one with the largest total time
     dom_name, dom = max(prof32.items(), key=lambda kv: kv[1]["total_ms"])
     roofline = {"bound": "hbm", "achieved": round(dom["GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
-                "avg_us": round(dom["avg_us"], 3), "bytes_per_launch": dom["bytes_per_launch"]}
+                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom_name),
+                "kernel": dom_name, "avg_us": round(dom["avg_us"], 3), "bytes_per_launch": dom["bytes_per_launch"]}
     step_bytes = sum(v["bytes_per_launch"] * v["count"] for v in prof32.values()) / a.profile_steps
 
     adj = None

@@ -107,6 +107,65 @@ def adjoint_timing(ff, torch, n=512, nu=5e-4, dt=0.002, nsteps=500, ncheck=23):
             "gradient_ms": round(tg, 3), "gradient_over_forward": round(tg / tf, 3)}
 
 
+def ensemble_timing(ff, torch, nsteps=500, dt=0.002):
+    """The mixed-grid ensemble: 64 FP32 members (32 x 128^2, 24 x 256^2,
+    8 x 512^2), own seed, nu log-uniform in [1e-4, 1e-2] and spectrum peak
+    k0 uniform in [4, 10] (E ~ k^4 exp(-2 (k/k0)^2)), ETDRK4 dt = 0.002,
+    `nsteps` steps in one run (autotuned variant, HIP-graph replays)."""
+    import numpy as np
+
+    members = ff.ensemble_members()
+    ens = ff.VorticityEnsemble([dict(nx=n, nu=nu) for (n, sd, nu, k0) in members], dt, T=np.float32,
+                               autotune=True)
+    for s, g, (n, sd, nu, k0) in zip(ens.sols, ens.grids, members):
+        s.copy_(ff.vorticity_ic(g, seed=sd, k0=k0, spectrum="k4exp2"))
+    ens.run(20)  # tunes, captures the graph
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    ens.run(nsteps)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / 1000.0
+    threads, prule, tuned_ms = ens.info()
+    return {"workload": "64 FP32 members: 32 x 128^2, 24 x 256^2, 8 x 512^2, ETDRK4 dt=0.002",
+            "steps": nsteps, "seconds": round(t, 4), "ensemble_steps_per_s": round(nsteps / t, 2),
+            "realization_steps_per_s": round(len(members) * nsteps / t, 1),
+            "grid_points_per_step": int(sum(n * n for (n, _, _, _) in members)),
+            "variant": {"threads": threads, "points_rule": prule, "tuned_ms_per_step": round(tuned_ms, 4)}}
+
+
+def tracer_timing(ff, torch, n=256, nsteps=300, dt=0.0025, kappa=1e-3):
+    """3D ABC-flow tracer, 256^3 FP32, RK4: forward steps/s and the cost of the
+    adjoint gradient of 1/2 ||c(T) - c_obs||^2 (forward + RK4 adjoint)."""
+    import numpy as np
+
+    p = ff.TracerProblem(n=n, kappa=kappa, dt=dt, T=np.float32)
+    g = p.grid
+    c0 = ff.tracer_ic(g, seed=1, ngauss=16, sigma=0.3)
+    cobs = ff.tracer_ic(g, seed=2, ngauss=16, sigma=0.3)
+    s = torch.zeros_like(p.sol)
+    ff.mul(s, g.rfftplan, c0)
+    ff.dealias(s, g)
+    p.sol.copy_(s)
+    ff.stepforward(p, 2)
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    p.sol.copy_(s)
+    e[0].record()
+    ff.stepforward(p, nsteps)
+    e[1].record()
+    e[2].record()
+    J, grad = ff.adjoint.tracer_misfit_gradient(p, c0, cobs, nsteps)
+    e[3].record()
+    torch.cuda.synchronize()
+    tf, tg = e[0].elapsed_time(e[1]) / 1000.0, e[2].elapsed_time(e[3]) / 1000.0
+    return {"workload": f"{n}^3 FP32 ABC tracer, RK4 dt={dt}, kappa={kappa}, {nsteps} steps",
+            "steps_per_s": round(nsteps / tf, 2), "forward_s": round(tf, 4), "gradient_s": round(tg, 4),
+            "gradient_over_forward": round(tg / tf, 3),
+            "note": "dt=0.01 is outside RK4's stability region for this flow at 256^3 (growth 2.4 per step)"}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes (scaled by the counters' reading of a 256 MiB copy), or None."""
@@ -158,6 +217,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost measurement")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the mixed-grid ensemble measurement")
+    ap.add_argument("--no-tracer", action="store_true", help="skip the 256^3 tracer measurement")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE)")
@@ -230,6 +291,15 @@ def main():
         adj = adjoint_timing(ff, torch)
         print(f"# adjoint: {adj}", file=sys.stderr, flush=True)
 
+    ens = None
+    if not a.no_ensemble:
+        ens = ensemble_timing(ff, torch)
+        print(f"# ensemble: {ens}", file=sys.stderr, flush=True)
+    trc = None
+    if not a.no_tracer:
+        trc = tracer_timing(ff, torch)
+        print(f"# tracer: {trc}", file=sys.stderr, flush=True)
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(a.cpu_seconds)
@@ -245,6 +315,8 @@ def main():
         "fp32_rel_l2_vs_fp64": {"steps": nacc, "field": err, "energy": abs(e32 - e64) / abs(e64),
                                 "enstrophy": abs(z32 - z64) / abs(z64)},
         "adjoint": adj,
+        "ensemble": ens,
+        "tracer": trc,
         "step_algorithmic_GB": round(step_bytes / 1e9, 4),
         "step_GBps": round(step_bytes / (t32 / a.steps) / 1e9, 1),
         "roofline": roofline,

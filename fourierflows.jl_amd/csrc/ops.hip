@@ -373,6 +373,7 @@ static int etd_run_generic(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t ns
     FF_TRY(etd_substep3(ts, s2, s1, N1, N3, st));
     FF_TRY(op_calcN(op, N4, s2, st));
     FF_TRY(etd_update(ts, sol, N1, N2, N3, N4, st));
+    FF_TRY(apply_filter(ts->prec, ts->count, ts->filter, sol, st));
   }
   return FFHIP_OK;
 }
@@ -398,6 +399,49 @@ static int rk4_run_generic(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, in
     FF_TRY(op_calcN(op, R4, s1, st));
     FF_TRY(rk4_addlinear(ts, R4, s1, st));
     FF_TRY(rk4_update(ts, sol, R1, R2, R3, R4, dt, st));
+    FF_TRY(apply_filter(ts->prec, ts->count, ts->filter, sol, st));
+  }
+  return FFHIP_OK;
+}
+
+// LSRK54 coefficients (src/timesteppers.jl:330-345), Carpenter & Kennedy (1994)
+static const double kLsrkA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                                 -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+static const double kLsrkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                                 1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                                 2277821191437.0 / 14882151754819.0};
+
+// ForwardEuler (kind 0), AB3 (kind 1) and LSRK54 (kind 2), filtered when ts
+// carries a filter, with a built-in operator (src/timesteppers.jl:112-149,
+// 389-414, 630-666).  step0 = clock.step at entry (AB3 starts with 3 Euler
+// steps); AB3's RHS_-1 and RHS_-2 live in the caller's arrays m1, m2.
+static int explicit_run_generic(ffhip_op* op, ffhip_rk4* ts, int kind, void* sol, double dt, int64_t nsteps,
+                                int64_t step0, void* m1, void* m2, hipStream_t st) {
+  FF_TRY(rk4_scratch(ts));
+  void* A = ts->scratch[1];
+  const size_t bytes = (size_t)ts->count * cplx_size(ts->prec);
+  const bool f32 = op->prec == FFHIP_F32;
+  for (int64_t k = 0; k < nsteps; ++k) {
+    if (kind == 0) {
+      FF_TRY(op_calcN(op, A, sol, st));
+      FF_TRY(euler_update(ts, sol, A, dt, st));
+    } else if (kind == 1) {
+      FF_TRY(op_calcN(op, A, sol, st));
+      FF_TRY(rk4_addlinear(ts, A, sol, st));
+      FF_TRY(ab3_update(ts, sol, A, m1, m2, dt, step0 + k, st));
+      FF_HIP(hipMemcpyAsync(m2, m1, bytes, hipMemcpyDeviceToDevice, st));  // RHS_-2 = RHS_-1
+      FF_HIP(hipMemcpyAsync(m1, A, bytes, hipMemcpyDeviceToDevice, st));   // RHS_-1 = RHS
+    } else {
+      void* S2 = ts->scratch[4];  // the first stage (A = 0) overwrites it
+      for (int i = 0; i < 5; ++i) {
+        FF_TRY(op_calcN(op, A, sol, st));
+        FF_TRY(rk4_addlinear(ts, A, sol, st));
+        const double a = f32 ? (double)(float)kLsrkA[i] : kLsrkA[i];
+        const double b = f32 ? (double)(float)kLsrkB[i] : kLsrkB[i];
+        FF_TRY(lsrk54_stage(ts, sol, S2, A, a, b, dt, st));
+      }
+      FF_TRY(apply_filter(ts->prec, ts->count, ts->filter, sol, st));
+    }
   }
   return FFHIP_OK;
 }
@@ -647,6 +691,21 @@ int ffhip_op_rk4_adjoint(ffhip_op* op, ffhip_rk4* ts, const void* lamT, void* la
   const double h = op->prec == FFHIP_F32 ? (double)(float)dt : dt;
   return op->prec == FFHIP_F32 ? rk4_adjoint_tracer<float>(op, ts, lam0, h, nsteps, st)
                                : rk4_adjoint_tracer<double>(op, ts, lam0, h, nsteps, st);
+}
+
+int ffhip_op_explicit_run(ffhip_op* op, ffhip_rk4* ts, int kind, void* sol, double dt, int64_t nsteps,
+                          int64_t step0, void* rhs_m1, void* rhs_m2, void* stream) {
+  clear_error();
+  FF_CHECK(op && ts && sol, FFHIP_ERR_ARG, "null argument");
+  FF_CHECK(kind >= 0 && kind <= 2, FFHIP_ERR_ARG, "explicit stepper kind must be 0 (ForwardEuler), 1 (AB3) or "
+                                                  "2 (LSRK54), got %d", kind);
+  FF_CHECK(nsteps >= 0 && step0 >= 0, FFHIP_ERR_ARG, "nsteps and step0 must be >= 0");
+  FF_CHECK(same_layout(op, ts->ndim, ts->dims, ts->prec), FFHIP_ERR_ARG,
+           "stepper shape/precision does not match the operator's rfft layout");
+  FF_CHECK(kind != 1 || (rhs_m1 && rhs_m2 && rhs_m1 != rhs_m2), FFHIP_ERR_ARG,
+           "AB3 needs two distinct arrays for RHS_-1 and RHS_-2");
+  const double h = op->prec == FFHIP_F32 ? (double)(float)dt : dt;
+  return explicit_run_generic(op, ts, kind, sol, h, nsteps, step0, rhs_m1, rhs_m2, as_stream(stream));
 }
 
 int ffhip_op_is_fused(const ffhip_op* op, int* fused) {

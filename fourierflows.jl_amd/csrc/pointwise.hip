@@ -96,6 +96,57 @@ __global__ void k_rk4_update(cplx<R>* sol, const cplx<R>* R1, const cplx<R>* R2,
   }
 }
 
+// sol *= filter (FP64 filter, rounded once to the state type)
+template <typename R>
+__global__ void k_filter(cplx<R>* sol, const double* f, int64_t n) {
+  FF_GRID_LOOP(i, n) { st2<R>(sol, i, d2scale(ld2<R>(sol, i), f[i])); }
+}
+
+// ForwardEuler (src/timesteppers.jl:112-118): sol += dt (L sol + N);
+// filtered (:142-149): sol = filter (sol + dt (N + L sol))
+template <typename R, bool CC>
+__global__ void k_euler(cplx<R>* sol, const cplx<R>* N, const void* L, const double* f, double dt, int64_t n) {
+  FF_GRID_LOOP(i, n) {
+    const double2 u = ld2<R>(sol, i);
+    double2 r = d2add(u, d2scale(d2add(cmulz<CC>(L, i, u), ld2<R>(N, i)), dt));
+    if (f) r = d2scale(r, f[i]);
+    st2<R>(sol, i, r);
+  }
+}
+
+// AB3update! (src/timesteppers.jl:630-638): forward Euler while step < 3, then
+// sol += dt (23/12 RHS - 16/12 RHS_-1 + 5/12 RHS_-2); then the filter if any
+template <typename R>
+__global__ void k_ab3(cplx<R>* sol, const cplx<R>* R0, const cplx<R>* R1, const cplx<R>* R2, const double* f,
+                      double dt, int euler, int64_t n) {
+  FF_GRID_LOOP(i, n) {
+    double2 r;
+    if (euler) {
+      r = d2scale(ld2<R>(R0, i), dt);
+    } else {
+      r = d2add(d2scale(ld2<R>(R0, i), 23.0 / 12.0), d2scale(ld2<R>(R1, i), -(16.0 / 12.0)));
+      r = d2scale(d2add(r, d2scale(ld2<R>(R2, i), 5.0 / 12.0)), dt);
+    }
+    cplx<R> u = sol[i];
+    u = mkc<R>((R)((double)u.x + r.x), (R)((double)u.y + r.y));
+    if (f) u = mkc<R>((R)((double)u.x * f[i]), (R)((double)u.y * f[i]));
+    sol[i] = u;
+  }
+}
+
+// LSRK54 stage (src/timesteppers.jl:389-397): S2 = A S2 + dt RHS; sol += B S2.
+// A == 0 (the first stage, after S2 = 0) does not read S2.
+template <typename R>
+__global__ void k_lsrk54(cplx<R>* sol, cplx<R>* S2, const cplx<R>* RHS, double A, double B, double dt, int64_t n) {
+  FF_GRID_LOOP(i, n) {
+    const double2 r = d2scale(ld2<R>(RHS, i), dt);
+    const double2 s = A == 0.0 ? r : d2add(d2scale(ld2<R>(S2, i), A), r);
+    st2<R>(S2, i, s);
+    const double2 sr = ld2<R>(S2, i);  // sol += B S2 with S2 in the state type
+    st2<R>(sol, i, d2add(ld2<R>(sol, i), d2scale(sr, B)));
+  }
+}
+
 // zero every entry whose index along some dimension lies in that dimension's
 // aliased range [lo, hi] (0-based, inclusive; empty when lo > hi)
 template <typename R>
@@ -322,6 +373,61 @@ static int upload_coef(void** d, const double* re, const double* im, int64_t cou
   return upload_f64(d, z.data(), 2 * count);
 }
 
+int apply_filter(int prec, int64_t count, const void* filter, void* sol, hipStream_t st) {
+  if (!filter) return FFHIP_OK;
+  if (prec == FFHIP_F32)
+    hipLaunchKernelGGL((k_filter<float>), dim3(pw_blocks(count)), dim3(kPwThreads), 0, st, (float2*)sol,
+                       (const double*)filter, count);
+  else
+    hipLaunchKernelGGL((k_filter<double>), dim3(pw_blocks(count)), dim3(kPwThreads), 0, st, (double2*)sol,
+                       (const double*)filter, count);
+  FF_LAUNCHED("k_filter");
+  return FFHIP_OK;
+}
+
+int euler_update(const ffhip_rk4* ts, void* sol, const void* N, double dt, hipStream_t st) {
+  FF_CHECK(ts && sol && N, FFHIP_ERR_ARG, "null argument");
+  const int64_t n = ts->count;
+#define LE(R, CC, ...)                                                                                     \
+  hipLaunchKernelGGL((k_euler<R, CC>), dim3(pw_blocks(n)), dim3(kPwThreads), 0, st, (cplx<R>*)sol,          \
+                     (const cplx<R>*)N, ts->L, (const double*)ts->filter, dt, n)
+  FF_DISPATCH_RC(ts->prec, ts->cplx_coef, LE, 0);
+#undef LE
+  FF_LAUNCHED("k_euler");
+  return FFHIP_OK;
+}
+
+int ab3_update(const ffhip_rk4* ts, void* sol, const void* RHS, const void* RHSm1, const void* RHSm2, double dt,
+               int64_t step, hipStream_t st) {
+  FF_CHECK(ts && sol && RHS && RHSm1 && RHSm2, FFHIP_ERR_ARG, "null argument");
+  const int64_t n = ts->count;
+  const int euler = step < 3;
+  if (ts->prec == FFHIP_F32)
+    hipLaunchKernelGGL((k_ab3<float>), dim3(pw_blocks(n)), dim3(kPwThreads), 0, st, (float2*)sol,
+                       (const float2*)RHS, (const float2*)RHSm1, (const float2*)RHSm2, (const double*)ts->filter,
+                       dt, euler, n);
+  else
+    hipLaunchKernelGGL((k_ab3<double>), dim3(pw_blocks(n)), dim3(kPwThreads), 0, st, (double2*)sol,
+                       (const double2*)RHS, (const double2*)RHSm1, (const double2*)RHSm2,
+                       (const double*)ts->filter, dt, euler, n);
+  FF_LAUNCHED("k_ab3");
+  return FFHIP_OK;
+}
+
+int lsrk54_stage(const ffhip_rk4* ts, void* sol, void* S2, const void* RHS, double A, double B, double dt,
+                 hipStream_t st) {
+  FF_CHECK(ts && sol && S2 && RHS, FFHIP_ERR_ARG, "null argument");
+  const int64_t n = ts->count;
+  if (ts->prec == FFHIP_F32)
+    hipLaunchKernelGGL((k_lsrk54<float>), dim3(pw_blocks(n)), dim3(kPwThreads), 0, st, (float2*)sol, (float2*)S2,
+                       (const float2*)RHS, A, B, dt, n);
+  else
+    hipLaunchKernelGGL((k_lsrk54<double>), dim3(pw_blocks(n)), dim3(kPwThreads), 0, st, (double2*)sol,
+                       (double2*)S2, (const double2*)RHS, A, B, dt, n);
+  FF_LAUNCHED("k_lsrk54");
+  return FFHIP_OK;
+}
+
 // scratch arrays are allocated on first use (callers that bring their own
 // arrays, and the fused operators, never touch them)
 int etd_scratch(ffhip_etdrk4* t) {
@@ -345,6 +451,7 @@ static void etd_free(ffhip_etdrk4* t) {
   for (auto& p : t->coef) dev_free(p);
   for (auto& p : t->scratch) dev_free(p);
   dev_free(t->packed);
+  dev_free(t->filter);
   delete[] t->L_host_re;
   delete[] t->L_host_im;
   delete t;
@@ -353,6 +460,7 @@ static void etd_free(ffhip_etdrk4* t) {
 static void rk4_free(ffhip_rk4* t) {
   if (!t) return;
   dev_free(t->L);
+  dev_free(t->filter);
   for (auto& p : t->scratch) dev_free(p);
   delete t;
 }
@@ -547,6 +655,58 @@ int ffhip_rk4_update(ffhip_rk4* ts, void* sol, const void* R1, const void* R2, c
                      double dt, void* stream) {
   clear_error();
   return rk4_update(ts, sol, R1, R2, R3, R4, dt, as_stream(stream));
+}
+
+static int set_filter(void** slot, const double* filter, int64_t count) {
+  dev_free(*slot);
+  *slot = nullptr;
+  if (!filter) return FFHIP_OK;
+  for (int64_t i = 0; i < count; ++i)
+    FF_CHECK(std::isfinite(filter[i]), FFHIP_ERR_ARG, "filter entry %lld is not finite", (long long)i);
+  return upload_f64(slot, filter, count);
+}
+
+int ffhip_etdrk4_set_filter(ffhip_etdrk4* ts, const double* filter) {
+  clear_error();
+  FF_CHECK(ts, FFHIP_ERR_ARG, "null stepper");
+  FF_TRY(set_filter(&ts->filter, filter, ts->count));
+  ts->packed_key = 0;  // fused paths repack their coefficient copies
+  return FFHIP_OK;
+}
+
+int ffhip_rk4_set_filter(ffhip_rk4* ts, const double* filter) {
+  clear_error();
+  FF_CHECK(ts, FFHIP_ERR_ARG, "null stepper");
+  return set_filter(&ts->filter, filter, ts->count);
+}
+
+int ffhip_etdrk4_apply_filter(ffhip_etdrk4* ts, void* sol, void* stream) {
+  clear_error();
+  FF_CHECK(ts && sol, FFHIP_ERR_ARG, "null argument");
+  return apply_filter(ts->prec, ts->count, ts->filter, sol, as_stream(stream));
+}
+
+int ffhip_rk4_apply_filter(ffhip_rk4* ts, void* sol, void* stream) {
+  clear_error();
+  FF_CHECK(ts && sol, FFHIP_ERR_ARG, "null argument");
+  return apply_filter(ts->prec, ts->count, ts->filter, sol, as_stream(stream));
+}
+
+int ffhip_euler_update(ffhip_rk4* ts, void* sol, const void* N, double dt, void* stream) {
+  clear_error();
+  return euler_update(ts, sol, N, dt, as_stream(stream));
+}
+
+int ffhip_ab3_update(ffhip_rk4* ts, void* sol, const void* RHS, const void* RHSm1, const void* RHSm2, double dt,
+                     int64_t step, void* stream) {
+  clear_error();
+  return ab3_update(ts, sol, RHS, RHSm1, RHSm2, dt, step, as_stream(stream));
+}
+
+int ffhip_lsrk54_stage(ffhip_rk4* ts, void* sol, void* S2, const void* RHS, double A, double B, double dt,
+                       void* stream) {
+  clear_error();
+  return lsrk54_stage(ts, sol, S2, RHS, A, B, dt, as_stream(stream));
 }
 
 }  // extern "C"

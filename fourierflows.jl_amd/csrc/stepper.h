@@ -24,6 +24,7 @@ struct ffhip_etdrk4 {
   // host copy of the linear operator (FP64) for the operators that need it
   double* L_host_re = nullptr;
   double* L_host_im = nullptr;
+  void* filter = nullptr;  // FilteredETDRK4: FP64 real spectral filter, or null
 };
 
 struct ffhip_rk4 {
@@ -34,6 +35,7 @@ struct ffhip_rk4 {
   int cplx_coef = 0;
   void* L = nullptr;  // FP64 real or complex
   void* scratch[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // sol1, RHS1..RHS4
+  void* filter = nullptr;  // Filtered* explicit steppers: FP64 real spectral filter, or null
 };
 
 namespace ffhip {
@@ -48,6 +50,13 @@ int rk4_addlinear(const ffhip_rk4* ts, void* RHS, const void* sol, hipStream_t s
 int rk4_substepsol(const ffhip_rk4* ts, void* newsol, const void* sol, const void* RHS, double dt, hipStream_t st);
 int rk4_update(const ffhip_rk4* ts, void* sol, const void* R1, const void* R2, const void* R3, const void* R4,
                double dt, hipStream_t st);
+// explicit steppers sharing the RK4 object (L, filter, scratch)
+int apply_filter(int prec, int64_t count, const void* filter, void* sol, hipStream_t st);
+int euler_update(const ffhip_rk4* ts, void* sol, const void* N, double dt, hipStream_t st);
+int ab3_update(const ffhip_rk4* ts, void* sol, const void* RHS, const void* RHSm1, const void* RHSm2, double dt,
+               int64_t step, hipStream_t st);
+int lsrk54_stage(const ffhip_rk4* ts, void* sol, void* S2, const void* RHS, double A, double B, double dt,
+                 hipStream_t st);
 int etd_scratch(ffhip_etdrk4* t);
 int rk4_scratch(ffhip_rk4* t);
 int dealias_launch(void* fh, int prec, int ndim, const int64_t* dims, int64_t batch, const int64_t* lo,

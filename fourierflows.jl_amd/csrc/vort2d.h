@@ -105,6 +105,7 @@ struct VortColArgs {
   const double* Ga;   // Gamma
   const void* tw;
   int do_inverse;
+  const double* Fl = nullptr;  // spectral filter applied after the update (FilteredETDRK4), or null
 };
 
 struct VortRowArgs {
@@ -126,8 +127,8 @@ template <typename R> inline size_t vfield_bytes(const VortGeom& g) {
 int vort_row(int prec, const VortGeom& g, const VortRowArgs& a, hipStream_t st);
 int vort_col(int prec, int stage, const VortGeom& g, const VortColArgs& a, hipStream_t st);
 int vort_import(int prec, const VortGeom& g, const void* sol, void* U, hipStream_t st);
-int vort_export(int prec, const VortGeom& g, void* sol, const void* U, const double* E, int64_t nsteps,
-                hipStream_t st);
+int vort_export(int prec, const VortGeom& g, void* sol, const void* U, const double* E, const double* F,
+                int64_t nsteps, hipStream_t st);
 int vort_ensure_packed(ffhip_op* op, ffhip_etdrk4* ts, const VortGeom& g, hipStream_t st);
 
 }  // namespace ffhip

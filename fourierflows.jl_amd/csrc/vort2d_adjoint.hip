@@ -51,6 +51,7 @@ struct VortAdjColArgs {
   const double* Ga;
   const void* tw;
   int cont;  // STAGE 4: 1 = start the next (earlier) step, 0 = store lambda_0 in LAM
+  const double* Fl;  // filter of a FilteredETDRK4 stepper (lambda <- filter lambda first), or null
 };
 
 struct VortAdjRowArgs {
@@ -81,8 +82,8 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vacol(VortGeom g, VortAdjColA
   const int ix = tx * CB + c;
   const bool colkept = ix < g.KX;
   const R kx = (R)(g.dk * (double)ix);
-  const int64_t ibase = (int64_t)tx * NY * CB + c;
-  const int64_t sbase = (int64_t)tx * g.KY * CB + c;
+  const uint32_t ibase = (uint32_t)tx * NY * CB + c;  // 32-bit offsets: uniform base + VGPR offset loads
+  const uint32_t sbase = (uint32_t)tx * g.KY * CB + c;
 
   // m = -i l/|k|^2 U + i kx/|k|^2 V + Z after the forward y-FFTs
   cplx<R> mb[P];
@@ -95,7 +96,7 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vacol(VortGeom g, VortAdjColA
     for (int f = 0; f < 3; ++f) {
       cplx<R> v[P];
 #pragma unroll
-      for (int p = 0; p < P; ++p) v[p] = O[f][ibase + (int64_t)(tid + p * T) * CB];
+      for (int p = 0; p < P; ++p) v[p] = O[f][ibase + (uint32_t)(tid + p * T) * CB];
       team_fft<NY, P, -1, R>(v, lds, tid, tw);
 #pragma unroll
       for (int p = 0; p < P; ++p) {
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vacol(VortGeom g, VortAdjColA
     const bool kept = colkept && (iy < g.ylo || iy > g.yhi);
     nx_[p] = mkc<R>(0, 0);
     if (!kept) continue;
-    const int64_t si = sbase + (int64_t)v_ky(iy, g) * CB;
+    const uint32_t si = sbase + (uint32_t)v_ky(iy, g) * CB;
     double lx = 0, ly = 0;  // lambda for the entry combination
     bool entry = false;
     if constexpr (STAGE == 0) {
@@ -169,6 +170,10 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vacol(VortGeom g, VortAdjColA
       }
     }
     if (entry) {
+      if (a.Fl) {  // transpose of sol .*= filter
+        lx *= a.Fl[si];
+        ly *= a.Fl[si];
+      }
       const double e = a.E[si], al = a.Al[si], be = 2.0 * a.Be[si], ga = a.Ga[si];
       UB[si] = mkc<R>((R)(e * lx), (R)(e * ly));
       N1B[si] = mkc<R>((R)(al * lx), (R)(al * ly));
@@ -193,7 +198,7 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vacol(VortGeom g, VortAdjColA
     team_fft<NY, P, 1, R>(v, lds, tid, tw);
     cplx<R>* d = dst[f];
 #pragma unroll
-    for (int p = 0; p < P; ++p) d[ibase + (int64_t)(tid + p * T) * CB] = v[p];
+    for (int p = 0; p < P; ++p) d[ibase + (uint32_t)(tid + p * T) * CB] = v[p];
   }
 }
 
@@ -212,7 +217,7 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_varow(VortGeom g, VortAdjRowA
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
   const int y = grp * 4 + sub * kRRK + r;
   const int64_t ny = g.ny;
-  auto iaddr = [&](int k) -> int64_t { return ((int64_t)(k / CB) * ny + y) * CB + (k % CB); };
+  auto iaddr = [&](int k) -> uint32_t { return ((uint32_t)(k / CB) * (uint32_t)ny + y) * CB + (k % CB); };
   const cplx<R>* Ph = reinterpret_cast<const cplx<R>*>(a.IA0);
   const cplx<R>* Qh = reinterpret_cast<const cplx<R>*>(a.IA1);
   const int KX = g.KX;
@@ -403,7 +408,7 @@ struct VortAdjPlan {
   int64_t nsteps = 0, ncheck = 0, seg = 0;
 };
 
-static VortColArgs fwd_col_args(VortState* s, const double* pc, size_t n) {
+static VortColArgs fwd_col_args(VortState* s, const double* pc, size_t n, bool filtered) {
   VortColArgs ca;
   ca.O0 = s->O[0];
   ca.O1 = s->O[1];
@@ -422,6 +427,7 @@ static VortColArgs fwd_col_args(VortState* s, const double* pc, size_t n) {
   ca.Eh = pc + 5 * n;
   ca.tw = s->twy;
   ca.do_inverse = 1;
+  ca.Fl = filtered ? pc + 6 * n : nullptr;
   return ca;
 }
 
@@ -467,7 +473,7 @@ static int vforward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void*
   const size_t n = (size_t)g.TX * g.KY * kVCB;
   const size_t sb = n * sizeof(cplx<R>);
   const double* pc = reinterpret_cast<const double*>(ts->packed);
-  VortColArgs ca = fwd_col_args(s, pc, n);
+  VortColArgs ca = fwd_col_args(s, pc, n, ts->filter != nullptr);
   VortRowArgs ra = fwd_row_args(s);
   const size_t fb = vfield_bytes<R>(g);
   FF_TRY(vort_import(prec, g, sol0, s->U, st));
@@ -479,7 +485,8 @@ static int vforward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void*
   }
   if (solT != sol0)
     FF_HIP(hipMemcpyAsync(solT, sol0, (size_t)g.ny * g.nkr * sizeof(cplx<R>), hipMemcpyDeviceToDevice, st));
-  FF_TRY(vort_export(prec, g, solT, s->U, reinterpret_cast<const double*>(ts->coef[4]), nsteps, st));
+  FF_TRY(vort_export(prec, g, solT, s->U, reinterpret_cast<const double*>(ts->coef[4]),
+                     reinterpret_cast<const double*>(ts->filter), nsteps, st));
   s->ck_ts = ts;
   s->ck_nsteps = nsteps;
   s->ck_ncheck = ncheck;
@@ -501,7 +508,7 @@ static int vadjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, void* lam0
   const size_t n = (size_t)g.TX * g.KY * kVCB;
   const size_t sb = n * sizeof(cplx<R>);
   const double* pc = reinterpret_cast<const double*>(ts->packed);
-  VortColArgs ca = fwd_col_args(s, pc, n);
+  VortColArgs ca = fwd_col_args(s, pc, n, ts->filter != nullptr);
   VortRowArgs ra = fwd_row_args(s);
   const size_t fb = vfield_bytes<R>(g);
 
@@ -524,6 +531,7 @@ static int vadjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, void* lam0
   aa.Eh = pc + 5 * n;
   aa.tw = s->twy;
   aa.cont = 1;
+  aa.Fl = ts->filter ? pc + 6 * n : nullptr;
   VortAdjRowArgs ar;
   ar.IA0 = s->IA[0];
   ar.IA1 = s->IA[1];
@@ -557,7 +565,8 @@ static int vadjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, void* lam0
   }
   if (lam0 != lamT)
     FF_HIP(hipMemcpyAsync(lam0, lamT, (size_t)g.ny * g.nkr * sizeof(cplx<R>), hipMemcpyDeviceToDevice, st));
-  FF_TRY(vort_export(prec, g, lam0, s->LAM, reinterpret_cast<const double*>(ts->coef[4]), nsteps, st));
+  FF_TRY(vort_export(prec, g, lam0, s->LAM, reinterpret_cast<const double*>(ts->coef[4]),
+                     reinterpret_cast<const double*>(ts->filter), nsteps, st));
   return FFHIP_OK;
 }
 

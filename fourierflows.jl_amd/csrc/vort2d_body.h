@@ -7,6 +7,77 @@
 
 namespace ffhip {
 
+// ETDRK4 stage combination at one kept mode (blocked index si) given this
+// stage's N there (src/timesteppers.jl:501-516, in FP64 with FP64
+// coefficients; 2 N3 - N1 and N2 + N3 formed in the state type as the
+// reference's broadcasts do).  STAGE 0 only reads the state.
+template <typename R, int STAGE>
+__device__ __forceinline__ cplx<R> vstage_point(const VortColArgs& a, const int64_t si, const cplx<R> n) {
+  cplx<R>* U = reinterpret_cast<cplx<R>*>(a.U);
+  cplx<R>* N1 = reinterpret_cast<cplx<R>*>(a.N1);
+  cplx<R>* N2 = reinterpret_cast<cplx<R>*>(a.N2);
+  cplx<R>* As = reinterpret_cast<cplx<R>*>(a.A);
+  if constexpr (STAGE == 0) {
+    return U[si];
+  } else if constexpr (STAGE == 1 || STAGE == 2) {
+    // ETDRK4substep12!: exp(dt L/2) u + zeta N
+    const cplx<R> u = U[si];
+    const double eh = a.Eh[si], z = a.Z[si];
+    const cplx<R> w = mkc<R>((R)(eh * (double)u.x + z * (double)n.x), (R)(eh * (double)u.y + z * (double)n.y));
+    if constexpr (STAGE == 1) {
+      N1[si] = n;
+      As[si] = w;
+    } else {
+      N2[si] = n;
+    }
+    return w;
+  } else if constexpr (STAGE == 3) {
+    // ETDRK4substep3!: exp(dt L/2) a + zeta (2 N3 - N1); keep N2 + N3 for the update
+    const cplx<R> av = As[si];
+    const cplx<R> n1 = N1[si];
+    const cplx<R> d = mkc<R>(R(2) * n.x - n1.x, R(2) * n.y - n1.y);
+    const double eh = a.Eh[si], z = a.Z[si];
+    const cplx<R> w = mkc<R>((R)(eh * (double)av.x + z * (double)d.x), (R)(eh * (double)av.y + z * (double)d.y));
+    const cplx<R> n2 = N2[si];
+    N2[si] = mkc<R>(n2.x + n.x, n2.y + n.y);
+    return w;
+  } else {
+    // ETDRK4update!: exp(dt L) u + alpha N1 + 2 beta (N2 + N3) + Gamma N4
+    const cplx<R> u = U[si];
+    const cplx<R> n1 = N1[si];
+    const cplx<R> s23 = N2[si];
+    const double e = a.E[si], al = a.Al[si], be = 2.0 * a.Be[si], ga = a.Ga[si];
+    double rx = e * (double)u.x + al * (double)n1.x;
+    double ry = e * (double)u.y + al * (double)n1.y;
+    rx = rx + be * (double)s23.x;
+    ry = ry + be * (double)s23.y;
+    rx = rx + ga * (double)n.x;
+    ry = ry + ga * (double)n.y;
+    cplx<R> w = mkc<R>((R)rx, (R)ry);
+    if (a.Fl) {  // sol .*= filter (src/timesteppers.jl:549-558), a second rounding as there
+      const double f = a.Fl[si];
+      w = mkc<R>((R)(f * (double)w.x), (R)(f * (double)w.y));
+    }
+    U[si] = w;
+    return w;
+  }
+}
+
+// i q z for the three inverse fields of the column pass: f = 0: zeta^,
+// f = 1: u^ = i l/|k|^2 zeta^, f = 2: v^ = -i kr/|k|^2 zeta^
+template <typename R>
+__device__ __forceinline__ cplx<R> vfield(int f, const cplx<R> z, const R kx, const R l, const bool origin) {
+  if (f == 0) return z;
+  const R K2 = kx * kx + l * l;
+  const R invK = origin ? R(0) : R(1) / K2;
+  if (f == 1) {
+    const R q = l * invK;
+    return mkc<R>(-q * z.y, q * z.x);
+  }
+  const R q = kx * invK;
+  return mkc<R>(q * z.y, -q * z.x);
+}
+
 // One y-column (kx index tx*4 + c) of the column pass, run by a team of NY/P
 // threads (team index tid) with its own LDS slice.  live = false marks a
 // padding column beyond the last tile: it joins the team barriers but reads
@@ -24,6 +95,10 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
   const int64_t sbase = (int64_t)tx * g.KY * CB + c;
 
   cplx<R> nn[P];
+  if constexpr (STAGE == 0) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) nn[p] = mkc<R>(0, 0);
+  }
   if constexpr (STAGE > 0) {
     cplx<R> vb[P];
     const cplx<R>* O0 = reinterpret_cast<const cplx<R>*>(a.O0);
@@ -45,10 +120,6 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
 
   // stage combination on the kept band
   cplx<R> nw[P];
-  cplx<R>* U = reinterpret_cast<cplx<R>*>(a.U);
-  cplx<R>* N1 = reinterpret_cast<cplx<R>*>(a.N1);
-  cplx<R>* N2 = reinterpret_cast<cplx<R>*>(a.N2);
-  cplx<R>* As = reinterpret_cast<cplx<R>*>(a.A);
 #pragma unroll FF_STAGE_UNROLL
   for (int p = 0; p < P; ++p) {
     const int iy = tid + p * T;
@@ -56,46 +127,7 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
     nw[p] = mkc<R>(0, 0);
     if (!kept) continue;
     const int64_t si = sbase + (int64_t)v_ky(iy, g) * CB;
-    if constexpr (STAGE == 0) {
-      nw[p] = U[si];
-    } else if constexpr (STAGE == 1 || STAGE == 2) {
-      // ETDRK4substep12!: exp(dt L/2) u + zeta N
-      const cplx<R> u = U[si];
-      const double eh = a.Eh[si], z = a.Z[si];
-      const cplx<R> n = nn[p];
-      nw[p] = mkc<R>((R)(eh * (double)u.x + z * (double)n.x), (R)(eh * (double)u.y + z * (double)n.y));
-      if constexpr (STAGE == 1) {
-        N1[si] = n;
-        As[si] = nw[p];
-      } else {
-        N2[si] = n;
-      }
-    } else if constexpr (STAGE == 3) {
-      // ETDRK4substep3!: exp(dt L/2) a + zeta (2 N3 - N1); keep N2 + N3 for the update
-      const cplx<R> av = As[si];
-      const cplx<R> n1 = N1[si];
-      const cplx<R> n3 = nn[p];
-      const cplx<R> d = mkc<R>(R(2) * n3.x - n1.x, R(2) * n3.y - n1.y);
-      const double eh = a.Eh[si], z = a.Z[si];
-      nw[p] = mkc<R>((R)(eh * (double)av.x + z * (double)d.x), (R)(eh * (double)av.y + z * (double)d.y));
-      const cplx<R> n2 = N2[si];
-      N2[si] = mkc<R>(n2.x + n3.x, n2.y + n3.y);
-    } else {
-      // ETDRK4update!: exp(dt L) u + alpha N1 + 2 beta (N2 + N3) + Gamma N4
-      const cplx<R> u = U[si];
-      const cplx<R> n1 = N1[si];
-      const cplx<R> s23 = N2[si];
-      const cplx<R> n4 = nn[p];
-      const double e = a.E[si], al = a.Al[si], be = 2.0 * a.Be[si], ga = a.Ga[si];
-      double rx = e * (double)u.x + al * (double)n1.x;
-      double ry = e * (double)u.y + al * (double)n1.y;
-      rx = rx + be * (double)s23.x;
-      ry = ry + be * (double)s23.y;
-      rx = rx + ga * (double)n4.x;
-      ry = ry + ga * (double)n4.y;
-      nw[p] = mkc<R>((R)rx, (R)ry);
-      U[si] = nw[p];
-    }
+    nw[p] = vstage_point<R, STAGE>(a, si, nn[p]);
   }
   if (!a.do_inverse) return;  // uniform over the block
 
@@ -108,21 +140,7 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       const int iy = tid + p * T;
-      if (f == 0) {
-        v[p] = nw[p];
-      } else {
-        const R l = (R)(g.dl * (double)vfreq(iy, NY));
-        const R K2 = kx * kx + l * l;
-        const R invK = (ix == 0 && iy == 0) ? R(0) : R(1) / K2;
-        const cplx<R> z = nw[p];
-        if (f == 1) {
-          const R q = l * invK;
-          v[p] = mkc<R>(-q * z.y, q * z.x);
-        } else {
-          const R q = kx * invK;
-          v[p] = mkc<R>(q * z.y, -q * z.x);
-        }
-      }
+      v[p] = vfield<R>(f, nw[p], kx, (R)(g.dl * (double)vfreq(iy, NY)), ix == 0 && iy == 0);
     }
     team_fft<NY, P, 1, R>(v, lds, tid, tw);
     cplx<R>* d = dst[f];
@@ -131,6 +149,83 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
       for (int p = 0; p < P; ++p) d[ibase + (int64_t)(tid + p * T) * CB] = v[p];
     }
   }
+}
+
+// The column pass of one y-column split over three teams of T = NY/P
+// threads (T a multiple of 64, so every wavefront belongs to one team and
+// the team branches below are wave-uniform).  Teams 0 and 1 run the two
+// forward FFTs side by side while team 2 only keeps the barrier count; all
+// threads then form N and the stage combination; each team then runs one of
+// the three inverse FFTs.  lds holds three team regions of
+// TeamSched<NY, P>::LDS slots.  The serial chain per column is two FFTs
+// instead of five.
+template <typename R, int NY, int P, int STAGE>
+__device__ __forceinline__ void vcol3_body(const VortGeom& g, const VortColArgs& a, const int tx, const int c,
+                                           const int team, const int tid, cplx<R>* lds) {
+  constexpr int T = NY / P;
+  constexpr int CB = kVCB;
+  constexpr int L = TeamSched<NY, P>::LDS;
+  constexpr int NBAR = 2 * (TeamSched<NY, P>::NPASS - 1);  // barriers inside one team_fft
+  static_assert(T % 64 == 0, "three-team column pass needs whole wavefronts per team");
+  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
+  cplx<R>* mine = lds + team * L;
+  cplx<R>* RA = lds;          // forward (u zeta)^ after the y-FFT, natural order
+  cplx<R>* RB = lds + L;      // forward (v zeta)^
+  cplx<R>* RW = lds + 2 * L;  // new stage state, natural order
+  const int ix = tx * CB + c;
+  const bool colkept = ix < g.KX;
+  const R kx = (R)(g.dk * (double)ix);
+  const uint32_t ibase = (uint32_t)tx * NY * CB + c;  // 32-bit offsets: uniform base + VGPR offset loads
+  const uint32_t sbase = (uint32_t)tx * g.KY * CB + c;
+  const int t = team * T + tid;  // thread index within the three teams
+
+  if constexpr (STAGE > 0) {
+    if (team < 2) {
+      const cplx<R>* O = reinterpret_cast<const cplx<R>*>(team == 0 ? a.O0 : a.O1);
+      cplx<R> v[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) v[p] = O[ibase + (uint32_t)(tid + p * T) * CB];
+      team_fft<NY, P, -1, R>(v, mine, tid, tw);
+#pragma unroll
+      for (int p = 0; p < P; ++p) mine[lpad(tid + p * T)] = v[p];
+    } else {
+#pragma unroll
+      for (int b = 0; b < NBAR; ++b) __syncthreads();
+    }
+    __syncthreads();
+  }
+
+  // N and the stage combination, all three teams over the y points
+#pragma unroll 2
+  for (int iy = t; iy < NY; iy += 3 * T) {
+    const bool kept = colkept && (iy < g.ylo || iy > g.yhi);
+    cplx<R> w = mkc<R>(0, 0);
+    if (kept) {
+      cplx<R> n = mkc<R>(0, 0);
+      if constexpr (STAGE > 0) {
+        const R l = (R)(g.dl * (double)vfreq(iy, NY));
+        const cplx<R> A = RA[lpad(iy)], B = RB[lpad(iy)];
+        n = mkc<R>(kx * A.y + l * B.y, -(kx * A.x) - l * B.x);  // -i kr A - i l B
+      }
+      w = vstage_point<R, STAGE>(a, sbase + (uint32_t)v_ky(iy, g) * CB, n);
+    }
+    RW[lpad(iy)] = w;
+  }
+  __syncthreads();
+  if (!a.do_inverse) return;  // uniform over the block
+
+  // team f: inverse y-FFT of field f (zeta^, u^, v^)
+  cplx<R> v[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int iy = tid + p * T;
+    v[p] = vfield<R>(team, RW[lpad(iy)], kx, (R)(g.dl * (double)vfreq(iy, NY)), ix == 0 && iy == 0);
+  }
+  __syncthreads();  // every team has read RW before team 2's transform reuses it
+  team_fft<NY, P, 1, R>(v, mine, tid, tw);
+  cplx<R>* d = reinterpret_cast<cplx<R>*>(team == 0 ? a.I0 : (team == 1 ? a.I1 : a.I2));
+#pragma unroll
+  for (int p = 0; p < P; ++p) d[ibase + (uint32_t)(tid + p * T) * CB] = v[p];
 }
 
 // One x-row y of the row pass, run by a team of NX/P threads (team index
@@ -230,6 +325,106 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
   }
 }
 
+// The row pass of two x-rows (y0, y0 + 1) by three teams of T = NX/P threads
+// (T a multiple of 64: wave-uniform team branches).  Team 0 inverts u + i v
+// of row y0, team 1 of row y0 + 1, team 2 zeta(y0) + i zeta(y0 + 1) -- three
+// complex transforms for the five real inverse fields of the two rows.
+// Teams 0 and 1 then form their row's products and run its forward
+// transform while team 2 keeps the barrier count.  Five transforms per two
+// rows instead of six, two in the serial chain instead of three.
+template <typename R, int NX, int P>
+__device__ __forceinline__ void vrow3_body(const VortGeom& g, const VortRowArgs& a, const int y0, const int team,
+                                           const int tid, cplx<R>* lds) {
+  constexpr int T = NX / P;
+  constexpr int CB = kVCB;
+  constexpr int L = TeamSched<NX, P>::LDS;
+  constexpr int NBAR = 2 * (TeamSched<NX, P>::NPASS - 1);  // barriers inside one team_fft
+  static_assert(T % 64 == 0, "three-team row pass needs whole wavefronts per team");
+  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
+  const int64_t ny = g.ny;
+  const int KX = g.KX;
+  cplx<R>* mine = lds + team * L;
+  cplx<R>* RZ = lds + 2 * L;  // zeta(y0) + i zeta(y0 + 1), natural order
+  const int y = y0 + (team & 1);
+  auto iaddr = [&](int k, int yy) -> uint32_t { return ((uint32_t)(k / CB) * (uint32_t)ny + yy) * CB + (k % CB); };
+
+  // team 0/1: u + i v of its row from (U, V) at row y; team 2: zeta(y0) + i zeta(y0 + 1)
+  // from (Z(y0), Z(y0 + 1)).  Both are A + i B with A, B Hermitian-extended half spectra.
+  const cplx<R>* SA = reinterpret_cast<const cplx<R>*>(team < 2 ? a.I1 : a.I0);
+  const cplx<R>* SB = reinterpret_cast<const cplx<R>*>(team < 2 ? a.I2 : a.I0);
+  const int ya = team < 2 ? y : y0, yb = team < 2 ? y : y0 + 1;
+  cplx<R> v[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int k = tid + p * T;
+    const bool lo = k < KX, hi = k > NX - KX;
+    const int kk = lo ? k : NX - k;
+    cplx<R> s = mkc<R>(0, 0);
+    if (lo || hi) {
+      cplx<R> A = SA[iaddr(kk, ya)], B = SB[iaddr(kk, yb)];
+      if (k == 0) A.y = B.y = R(0);  // c2r ignores Im at kx = 0
+      if (hi) {
+        A.y = -A.y;  // conj
+        B.y = -B.y;
+      }
+      s = mkc<R>(A.x - B.y, A.y + B.x);
+    }
+    v[p] = s;
+  }
+  team_fft<NX, P, 1, R>(v, mine, tid, tw);
+  if (team == 2) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) RZ[lpad(tid + p * T)] = v[p];
+  }
+  __syncthreads();
+
+  if (team < 2) {
+    const R s2 = (R)g.s2;
+    const R s1 = (R)(1.0 / ((double)g.nx * (double)g.ny));
+    cplx<R>* Fz = a.F ? reinterpret_cast<cplx<R>*>(a.F) + (int64_t)y * NX : nullptr;
+    R* Fr = a.F ? reinterpret_cast<R*>(reinterpret_cast<cplx<R>*>(a.F) + (int64_t)g.nx * g.ny) + (int64_t)y * NX
+                : nullptr;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const cplx<R> zz = RZ[lpad(tid + p * T)];
+      const R zr = team == 0 ? zz.x : zz.y;
+      if (Fz) {  // keep u + i v and zeta (normalised) for the adjoint sweep
+        Fz[tid + p * T] = mkc<R>(v[p].x * s1, v[p].y * s1);
+        Fr[tid + p * T] = zr * s1;
+      }
+      const R zs = zr * s2;
+      v[p] = mkc<R>(v[p].x * zs, v[p].y * zs);
+    }
+    team_fft<NX, P, -1, R>(v, mine, tid, tw);
+#pragma unroll
+    for (int p = 0; p < P; ++p) mine[lpad(tid + p * T)] = v[p];
+  } else {
+#pragma unroll
+    for (int b = 0; b < NBAR; ++b) __syncthreads();
+  }
+  __syncthreads();
+
+  if (team < 2) {
+    // split W into the half spectra of its real and imaginary parts
+    cplx<R>* O0 = reinterpret_cast<cplx<R>*>(a.O0);
+    cplx<R>* O1 = reinterpret_cast<cplx<R>*>(a.O1);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int k = tid + p * T;
+      if (k >= g.KXp) continue;
+      cplx<R> Pk = mkc<R>(0, 0), Qk = mkc<R>(0, 0);
+      if (k < KX) {
+        const cplx<R> wk = v[p];
+        const cplx<R> wc = cconj<R>(mine[lpad((NX - k) & (NX - 1))]);
+        Pk = mkc<R>(R(0.5) * (wk.x + wc.x), R(0.5) * (wk.y + wc.y));
+        Qk = mkc<R>(R(0.5) * (wk.y - wc.y), -R(0.5) * (wk.x - wc.x));  // (wk - wc) / (2i)
+      }
+      O0[iaddr(k, y)] = Pk;
+      O1[iaddr(k, y)] = Qk;
+    }
+  }
+}
+
 // ----------------------------------------------------------------- layout conversion
 template <typename R>
 __global__ void k_vimport(const cplx<R>* __restrict__ sol, cplx<R>* __restrict__ U, VortGeom g) {
@@ -259,10 +454,11 @@ static __global__ void k_vpack(const double* __restrict__ src, double* __restric
 }
 
 // Scatter the kept band back; modes outside it evolve as u <- exp(dt L) u per
-// step (N is zero there), done here as nsteps roundings in the state type.
+// step (N is zero there), then u <- filter u when F is set, done here as
+// nsteps roundings in the state type.
 template <typename R>
 __global__ void k_vexport(cplx<R>* __restrict__ sol, const cplx<R>* __restrict__ U, const double* __restrict__ E,
-                          int64_t nsteps, VortGeom g) {
+                          const double* __restrict__ F, int64_t nsteps, VortGeom g) {
   const int64_t total = (int64_t)g.ny * g.nkr;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
@@ -276,7 +472,11 @@ __global__ void k_vexport(cplx<R>* __restrict__ sol, const cplx<R>* __restrict__
       cplx<R> u = sol[idx];
       if (u.x != R(0) || u.y != R(0)) {
         const double e = E[idx];
-        for (int64_t s = 0; s < nsteps; ++s) u = mkc<R>((R)(e * (double)u.x), (R)(e * (double)u.y));
+        const double f = F ? F[idx] : 1.0;
+        for (int64_t s = 0; s < nsteps; ++s) {
+          u = mkc<R>((R)(e * (double)u.x), (R)(e * (double)u.y));
+          if (F) u = mkc<R>((R)(f * (double)u.x), (R)(f * (double)u.y));
+        }
         sol[idx] = u;
       }
     }

@@ -427,6 +427,7 @@ int ffhip_ensemble_create(ffhip_ensemble** out, int prec, int count, const int64
     ca.E = pc + 4 * nc;
     ca.Eh = pc + 5 * nc;
     ca.do_inverse = 1;
+    ca.Fl = nullptr;
     for (int len : {g.nx, g.ny}) {
       if (!e->tw.count(len)) {
         void* t = nullptr;
@@ -518,7 +519,7 @@ int ffhip_ensemble_run(ffhip_ensemble* e, void* const* sols, int64_t nsteps, voi
   for (; left > 0; --left) FF_TRY(ens_step(e, st));
   for (int m = 0; m < e->count; ++m)
     FF_TRY(vort_export(e->prec, e->geo[m], sols[m], e->hmem[m].ca.U, reinterpret_cast<const double*>(e->Efull[m]),
-                       nsteps, st));
+                       nullptr, nsteps, st));
   return FFHIP_OK;
 }
 

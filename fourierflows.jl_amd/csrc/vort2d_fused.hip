@@ -48,6 +48,36 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vcol(VortGeom g, VortColArgs 
   vcol_body<R, NY, P, STAGE>(g, a, tx, sub * kCBK + cc, t / kCBK, lds, true);
 }
 
+// One column per workgroup of three teams (vcol3_body); the four columns of a
+// tile run on the same XCD.
+template <typename R, int NY, int P, int STAGE>
+__global__ void __launch_bounds__(3 * (NY / P)) k_vcol3(VortGeom g, VortColArgs a) {
+  extern __shared__ unsigned char vsm[];
+  int tx, c;
+  xcd_split(blockIdx.x, kVCB, &tx, &c);
+  if (tx >= g.TX) return;  // grid is rounded up to whole XCD groups
+  constexpr int T = NY / P;
+  const int t = threadIdx.x;
+  vcol3_body<R, NY, P, STAGE>(g, a, tx, c, t / T, t % T, reinterpret_cast<cplx<R>*>(vsm));
+}
+
+// Two rows per workgroup of three teams (vrow3_body); the two workgroups of
+// one 4-row group run on the same XCD.
+#ifndef FF_ROW3_WAVES
+#define FF_ROW3_WAVES 3  // FP32 waves per SIMD the three-team row pass is compiled for
+#endif
+template <typename R, int NX, int P>
+__global__ void __launch_bounds__(3 * (NX / P), sizeof(R) == 4 ? FF_ROW3_WAVES : 2) k_vrow3(VortGeom g,
+                                                                                           VortRowArgs a) {
+  extern __shared__ unsigned char vsm[];
+  int grp, sub;
+  xcd_split(blockIdx.x, 2, &grp, &sub);
+  if (grp >= g.ny / 4) return;  // grid is rounded up to whole XCD groups
+  constexpr int T = NX / P;
+  const int t = threadIdx.x;
+  vrow3_body<R, NX, P>(g, a, grp * 4 + sub * 2, t / T, t % T, reinterpret_cast<cplx<R>*>(vsm));
+}
+
 // kRRK rows per workgroup; the kRSUB workgroups of one 4-row group run on the same XCD.
 template <typename R, int NX, int P>
 __global__ void __launch_bounds__(kRRK*(NX / P)) k_vrow(VortGeom g, VortRowArgs a) {
@@ -95,6 +125,63 @@ static int launch_col(const VortGeom& g, const VortColArgs& a, hipStream_t st) {
   return FFHIP_OK;
 }
 
+// Three-team passes (measured on MI355X at 2048^2: faster in FP64, slower in
+// FP32, where the single-team kernels keep two waves per SIMD busy).
+// FFHIP_COL3 / FFHIP_ROW3: 0 off, 1 FP64 only (default), 2 FP32 and FP64.
+static bool use3(const char* var, bool f32) {
+  const char* e = getenv(var);
+  const int mode = e ? atoi(e) : 1;
+  return f32 ? mode >= 2 : mode >= 1;
+}
+static bool col3_on(bool f32) {
+  static const bool on32 = use3("FFHIP_COL3", true), on64 = use3("FFHIP_COL3", false);
+  return f32 ? on32 : on64;
+}
+static bool row3_on(bool f32) {
+  static const bool on32 = use3("FFHIP_ROW3", true), on64 = use3("FFHIP_ROW3", false);
+  return f32 ? on32 : on64;
+}
+
+template <typename R, int N, int STAGE>
+static int launch_col3(const VortGeom& g, const VortColArgs& a, hipStream_t st) {
+  constexpr int P = vpts_col(N);
+  const size_t sh = (size_t)3 * TeamSched<N, P>::LDS * sizeof(cplx<R>);
+  auto kern = k_vcol3<R, N, P, STAGE>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    FF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)sh));
+    attr_set = true;
+  }
+  static const char* names[5] = {"vort_col_stage0", "vort_col_stage1", "vort_col_stage2", "vort_col_stage3",
+                                 "vort_col_stage4"};
+  prof_begin(names[STAGE], col_bytes<R>(g, STAGE, a.do_inverse), st);
+  const int groups = (g.TX + kXCD - 1) / kXCD * kXCD;
+  hipLaunchKernelGGL(kern, dim3(groups * kVCB), dim3(3 * (N / P)), sh, st, g, a);
+  prof_end(st);
+  FF_LAUNCHED("k_vcol3");
+  return FFHIP_OK;
+}
+
+template <typename R, int N>
+static int launch_row3(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
+  constexpr int P = vpts_row(N);
+  const size_t sh = (size_t)3 * TeamSched<N, P>::LDS * sizeof(cplx<R>);
+  auto kern = k_vrow3<R, N, P>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    FF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)sh));
+    attr_set = true;
+  }
+  prof_begin("vort_row", row_bytes<R>(g), st);
+  const int groups = (g.ny / 4 + kXCD - 1) / kXCD * kXCD;
+  hipLaunchKernelGGL(kern, dim3(groups * 2), dim3(3 * (N / P)), sh, st, g, a);
+  prof_end(st);
+  FF_LAUNCHED("k_vrow3");
+  return FFHIP_OK;
+}
+
 template <typename R, int N>
 static int launch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
   constexpr int P = vpts_row(N);
@@ -121,8 +208,12 @@ static int dispatch_col(const VortGeom& g, const VortColArgs& a, hipStream_t st)
     case 128: return launch_col<R, 128, STAGE>(g, a, st);
     case 256: return launch_col<R, 256, STAGE>(g, a, st);
     case 512: return launch_col<R, 512, STAGE>(g, a, st);
-    case 1024: return launch_col<R, 1024, STAGE>(g, a, st);
-    case 2048: return launch_col<R, 2048, STAGE>(g, a, st);
+    case 1024:
+      if (col3_on(sizeof(R) == 4)) return launch_col3<R, 1024, STAGE>(g, a, st);
+      return launch_col<R, 1024, STAGE>(g, a, st);
+    case 2048:
+      if (col3_on(sizeof(R) == 4)) return launch_col3<R, 2048, STAGE>(g, a, st);
+      return launch_col<R, 2048, STAGE>(g, a, st);
     default: return fail(FFHIP_ERR_UNSUPPORTED, "fused column kernel: ny = %d", g.ny);
   }
 }
@@ -134,8 +225,12 @@ static int dispatch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st)
     case 128: return launch_row<R, 128>(g, a, st);
     case 256: return launch_row<R, 256>(g, a, st);
     case 512: return launch_row<R, 512>(g, a, st);
-    case 1024: return launch_row<R, 1024>(g, a, st);
-    case 2048: return launch_row<R, 2048>(g, a, st);
+    case 1024:
+      if (row3_on(sizeof(R) == 4)) return launch_row3<R, 1024>(g, a, st);
+      return launch_row<R, 1024>(g, a, st);
+    case 2048:
+      if (row3_on(sizeof(R) == 4)) return launch_row3<R, 2048>(g, a, st);
+      return launch_row<R, 2048>(g, a, st);
     default: return fail(FFHIP_ERR_UNSUPPORTED, "fused row kernel: nx = %d", g.nx);
   }
 }
@@ -234,11 +329,13 @@ int vort_ensure_packed(ffhip_op* op, ffhip_etdrk4* ts, const VortGeom& g, hipStr
   dev_free(ts->packed);
   ts->packed = nullptr;
   const size_t n = (size_t)g.TX * g.KY * kVCB;
-  FF_TRY(dev_alloc(&ts->packed, 6 * n * sizeof(double)));
+  const int ncoef = ts->filter ? 7 : 6;  // zeta, alpha, beta, Gamma, exp(dt L), exp(dt L/2)[, filter]
+  FF_TRY(dev_alloc(&ts->packed, ncoef * n * sizeof(double)));
   double* dst = reinterpret_cast<double*>(ts->packed);
-  for (int c = 0; c < 6; ++c) {
+  for (int c = 0; c < ncoef; ++c) {
+    const void* src = c < 6 ? ts->coef[c] : ts->filter;
     hipLaunchKernelGGL(k_vpack, dim3(blocks_for((int64_t)n, 256, 4096)), dim3(256), 0, st,
-                       reinterpret_cast<const double*>(ts->coef[c]), dst + c * n, g);
+                       reinterpret_cast<const double*>(src), dst + c * n, g);
     FF_LAUNCHED("k_vpack");
   }
   ts->packed_owner = op;
@@ -274,6 +371,7 @@ static int vrun(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, hipSt
   ca.Eh = pc + 5 * n;
   ca.tw = s->twy;
   ca.do_inverse = 1;
+  ca.Fl = ts->filter ? pc + 6 * n : nullptr;
   VortRowArgs ra;
   ra.I0 = s->I[0];
   ra.I1 = s->I[1];
@@ -293,7 +391,8 @@ static int vrun(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, hipSt
   const int64_t total = (int64_t)g.ny * g.nkr;
   hipLaunchKernelGGL((k_vexport<R>), dim3(blocks_for(total, 256, 4096)), dim3(256), 0, st,
                      reinterpret_cast<cplx<R>*>(sol), reinterpret_cast<const cplx<R>*>(s->U),
-                     reinterpret_cast<const double*>(ts->coef[4]), nsteps, g);
+                     reinterpret_cast<const double*>(ts->coef[4]), reinterpret_cast<const double*>(ts->filter),
+                     nsteps, g);
   FF_LAUNCHED("k_vexport");
   return FFHIP_OK;
 }
@@ -318,15 +417,15 @@ int vort_import(int prec, const VortGeom& g, const void* sol, void* U, hipStream
   return FFHIP_OK;
 }
 
-int vort_export(int prec, const VortGeom& g, void* sol, const void* U, const double* E, int64_t nsteps,
-                hipStream_t st) {
+int vort_export(int prec, const VortGeom& g, void* sol, const void* U, const double* E, const double* F,
+                int64_t nsteps, hipStream_t st) {
   const int64_t total = (int64_t)g.ny * g.nkr;
   if (prec == FFHIP_F32)
     hipLaunchKernelGGL((k_vexport<float>), dim3(blocks_for(total, 256, 4096)), dim3(256), 0, st,
-                       reinterpret_cast<float2*>(sol), reinterpret_cast<const float2*>(U), E, nsteps, g);
+                       reinterpret_cast<float2*>(sol), reinterpret_cast<const float2*>(U), E, F, nsteps, g);
   else
     hipLaunchKernelGGL((k_vexport<double>), dim3(blocks_for(total, 256, 4096)), dim3(256), 0, st,
-                       reinterpret_cast<double2*>(sol), reinterpret_cast<const double2*>(U), E, nsteps, g);
+                       reinterpret_cast<double2*>(sol), reinterpret_cast<const double2*>(U), E, F, nsteps, g);
   FF_LAUNCHED("k_vexport");
   return FFHIP_OK;
 }

@@ -99,6 +99,14 @@ SIGNATURES = {
     "ffhip_op_etdrk4_forward_ckpt": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
     "ffhip_op_etdrk4_adjoint": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
     "ffhip_op_rk4_adjoint": [_vp, _vp, _vp, _vp, _dbl, _i64, _vp],
+    "ffhip_op_explicit_run": [_vp, _vp, _int, _vp, _dbl, _i64, _i64, _vp, _vp, _vp],
+    "ffhip_etdrk4_set_filter": [_vp, _dp],
+    "ffhip_rk4_set_filter": [_vp, _dp],
+    "ffhip_etdrk4_apply_filter": [_vp, _vp, _vp],
+    "ffhip_rk4_apply_filter": [_vp, _vp, _vp],
+    "ffhip_euler_update": [_vp, _vp, _vp, _dbl, _vp],
+    "ffhip_ab3_update": [_vp, _vp, _vp, _vp, _vp, _dbl, _i64, _vp],
+    "ffhip_lsrk54_stage": [_vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _vp],
     "ffhip_ensemble_create": [C.POINTER(_vp), _int, _int, _i64p, _dp, _dbl, C.POINTER(_dp), _dbl, _int],
     "ffhip_ensemble_destroy": [_vp],
     "ffhip_ensemble_run": [_vp, C.POINTER(_vp), _i64, _vp],

@@ -188,22 +188,197 @@ class RK4TimeStepper(_Handle):
         clock.step += nsteps
 
 
-_STEPPERS = {"ETDRK4": ETDRK4TimeStepper, "RK4": RK4TimeStepper}
-fullyexplicitsteppers = ["RK4"]
+def _attach_filter(ts, eqn, setter, filterkw):
+    """makefilter(equation; kw...) (src/domains.jl:542) uploaded to the stepper."""
+    from .domains import makefilter
+
+    g = eqn.grid
+    f = makefilter(g, realvars=eqn.dims[0] == g.nkr, dims=eqn.dims, **filterkw)
+    ts.filter_host = f
+    _lib.call(setter, ts.handle, _lib.dptr(np.ascontiguousarray(f, dtype=np.float64)))
+
+
+class FilteredETDRK4TimeStepper(ETDRK4TimeStepper):
+    """src/timesteppers.jl:471-499, 549-558: ETDRK4, then ``sol .*= filter``
+    (inside the fused update kernel when the operator is built in)."""
+
+    def __init__(self, eqn, dt, dev=None, **filterkw):
+        super().__init__(eqn, dt, dev)
+        _attach_filter(self, eqn, "ffhip_etdrk4_set_filter", filterkw)
+
+    def stepforward(self, sol, clock, eqn, vars, params, grid):
+        t = clock.t
+        eqn.calcN(self.N1, sol, t, clock, vars, params, grid)
+        self.substep12(self.sol1, sol, self.N1)
+        t2 = clock.t + clock.dt / 2
+        eqn.calcN(self.N2, self.sol1, t2, clock, vars, params, grid)
+        self.substep12(self.sol2, sol, self.N2)
+        eqn.calcN(self.N3, self.sol2, t2, clock, vars, params, grid)
+        self.substep3(self.sol2, self.sol1, self.N1, self.N3)
+        eqn.calcN(self.N4, self.sol2, clock.t + clock.dt, clock, vars, params, grid)
+        self.update(sol, self.N1, self.N2, self.N3, self.N4)
+        _lib.call("ffhip_etdrk4_apply_filter", self._h, ptr(sol), stream_ptr())
+        clock.t += clock.dt
+        clock.step += 1
+
+
+class FilteredRK4TimeStepper(RK4TimeStepper):
+    """src/timesteppers.jl:200-223, 276-285: RK4, then ``sol .*= filter``."""
+
+    def __init__(self, eqn, dt=None, dev=None, **filterkw):
+        super().__init__(eqn, dt, dev)
+        _attach_filter(self, eqn, "ffhip_rk4_set_filter", filterkw)
+
+    def stepforward(self, sol, clock, eqn, vars, params, grid):
+        super().stepforward(sol, clock, eqn, vars, params, grid)
+        _lib.call("ffhip_rk4_apply_filter", self._h, ptr(sol), stream_ptr())
+
+
+class _ExplicitBase(RK4TimeStepper):
+    """ForwardEuler / AB3 / LSRK54 share the RK4 object of libffhip (L in FP64,
+    optional filter); the scratch arrays are the library's own."""
+
+    kind = -1
+
+    def __init__(self, eqn, dt=None, dev=None, **filterkw):
+        dev = eqn.grid.device if dev is None else dev
+        self.T = eqn.T
+        self.dims = eqn.dims
+        Lre, Lim = _host_L(eqn.L, eqn.dims)
+        h = C.c_void_p()
+        _lib.call("ffhip_rk4_create", C.byref(h), _lib.dptr(Lre), _lib.dptr(Lim), len(self.dims),
+                  _lib.i64arr(self.dims), prec_of(eqn.T))
+        self._h = h
+        self.filtered = False
+        if filterkw.pop("_filtered", False):
+            _attach_filter(self, eqn, "ffhip_rk4_set_filter", filterkw)
+            self.filtered = True
+
+    def _history(self):
+        return None, None
+
+    def run_fused(self, op, sol, clock, nsteps):
+        m1, m2 = self._history()
+        _lib.call("ffhip_op_explicit_run", op.handle, self._h, self.kind, ptr(sol), float(clock.dt), int(nsteps),
+                  int(clock.step), ptr(m1) if m1 is not None else None, ptr(m2) if m2 is not None else None,
+                  stream_ptr())
+        clock.t += nsteps * clock.dt
+        clock.step += nsteps
+
+
+class ForwardEulerTimeStepper(_ExplicitBase):
+    """src/timesteppers.jl:98-118: ``sol += dt (L sol + N)``."""
+
+    kind = 0
+
+    def __init__(self, eqn, dt=None, dev=None, **kw):
+        super().__init__(eqn, dt, dev, **kw)
+        self.N = zeros(eqn.grid.device if dev is None else dev, eqn.T, eqn.dims)
+
+    def stepforward(self, sol, clock, eqn, vars, params, grid):
+        eqn.calcN(self.N, sol, clock.t, clock, vars, params, grid)
+        _lib.call("ffhip_euler_update", self._h, ptr(sol), ptr(self.N), float(clock.dt), stream_ptr())
+        clock.t += clock.dt
+        clock.step += 1
+
+
+class AB3TimeStepper(_ExplicitBase):
+    """src/timesteppers.jl:588-650: forward Euler while ``clock.step < 3``, then
+    third-order Adams-Bashforth; RHS, RHS_-1, RHS_-2 kept on the device."""
+
+    kind = 1
+
+    def __init__(self, eqn, dt=None, dev=None, **kw):
+        super().__init__(eqn, dt, dev, **kw)
+        dev = eqn.grid.device if dev is None else dev
+        z = lambda: zeros(dev, eqn.T, eqn.dims)  # noqa: E731
+        self.RHS, self.RHSm1, self.RHSm2 = z(), z(), z()
+
+    def stepforward(self, sol, clock, eqn, vars, params, grid):
+        eqn.calcN(self.RHS, sol, clock.t, clock, vars, params, grid)
+        self.addlinearterm(self.RHS, sol)
+        _lib.call("ffhip_ab3_update", self._h, ptr(sol), ptr(self.RHS), ptr(self.RHSm1), ptr(self.RHSm2),
+                  float(clock.dt), int(clock.step), stream_ptr())
+        clock.t += clock.dt
+        clock.step += 1
+        self.RHSm2.copy_(self.RHSm1)
+        self.RHSm1.copy_(self.RHS)
+
+    def _history(self):
+        return self.RHSm1, self.RHSm2
+
+
+# src/timesteppers.jl:330-345
+LSRK54_A = (0.0, -567301805773 / 1357537059087, -2404267990393 / 2016746695238, -3550918686646 / 2091501179385,
+            -1275806237668 / 842570457699)
+LSRK54_B = (1432997174477 / 9575080441755, 5161836677717 / 13612068292357, 1720146321549 / 2090206949498,
+            3134564353537 / 4481467310338, 2277821191437 / 14882151754819)
+LSRK54_C = (0.0, 1432997174477 / 9575080441755, 2526269341429 / 6820363962896, 2006345519317 / 3224310063776,
+            2802321613138 / 2924317926251)
+
+
+class LSRK54TimeStepper(_ExplicitBase):
+    """src/timesteppers.jl:318-414: five stages of S2 = A_i S2 + dt RHS(sol, t + C_i dt);
+    sol += B_i S2 (coefficients rounded to the state's float type as ``T[...]`` does)."""
+
+    kind = 2
+
+    def __init__(self, eqn, dt=None, dev=None, **kw):
+        super().__init__(eqn, dt, dev, **kw)
+        dev = eqn.grid.device if dev is None else dev
+        self.S2 = zeros(dev, eqn.T, eqn.dims)
+        self.RHS = zeros(dev, eqn.T, eqn.dims)
+        ft = np.dtype(fltype(eqn.T)).type
+        self.A = tuple(float(ft(a)) for a in LSRK54_A)
+        self.B = tuple(float(ft(b)) for b in LSRK54_B)
+        self.C = tuple(float(ft(c)) for c in LSRK54_C)
+
+    def stepforward(self, sol, clock, eqn, vars, params, grid):
+        t, dt = clock.t, clock.dt
+        for i in range(5):
+            eqn.calcN(self.RHS, sol, t + self.C[i] * dt, clock, vars, params, grid)
+            self.addlinearterm(self.RHS, sol)
+            _lib.call("ffhip_lsrk54_stage", self._h, ptr(sol), ptr(self.S2), ptr(self.RHS), self.A[i], self.B[i],
+                      float(dt), stream_ptr())
+        if self.filtered:
+            _lib.call("ffhip_rk4_apply_filter", self._h, ptr(sol), stream_ptr())
+        clock.t += clock.dt
+        clock.step += 1
+
+
+def _filtered(cls, name):
+    def init(self, eqn, dt=None, dev=None, **filterkw):
+        cls.__init__(self, eqn, dt, dev, _filtered=True, **filterkw)
+
+    return type(name, (cls,), {"__init__": init, "__doc__": f"Filtered {cls.__name__} (src/timesteppers.jl)."})
+
+
+FilteredForwardEulerTimeStepper = _filtered(ForwardEulerTimeStepper, "FilteredForwardEulerTimeStepper")
+FilteredAB3TimeStepper = _filtered(AB3TimeStepper, "FilteredAB3TimeStepper")
+FilteredLSRK54TimeStepper = _filtered(LSRK54TimeStepper, "FilteredLSRK54TimeStepper")
+
+
+_STEPPERS = {"ETDRK4": ETDRK4TimeStepper, "RK4": RK4TimeStepper, "ForwardEuler": ForwardEulerTimeStepper,
+             "AB3": AB3TimeStepper, "LSRK54": LSRK54TimeStepper, "FilteredETDRK4": FilteredETDRK4TimeStepper,
+             "FilteredRK4": FilteredRK4TimeStepper, "FilteredForwardEuler": FilteredForwardEulerTimeStepper,
+             "FilteredAB3": FilteredAB3TimeStepper, "FilteredLSRK54": FilteredLSRK54TimeStepper}
+# src/timesteppers.jl:672-681
+fullyexplicitsteppers = ["ForwardEuler", "RK4", "AB3", "LSRK54", "FilteredForwardEuler", "FilteredRK4",
+                         "FilteredAB3", "FilteredLSRK54"]
 
 
 def isexplicit(stepper):
     return stepper in fullyexplicitsteppers
 
 
-def TimeStepper(stepper, equation, dt=None, dev=None):
-    """src/timesteppers.jl:57-69 (string -> stepper)."""
+def TimeStepper(stepper, equation, dt=None, dev=None, **kw):
+    """src/timesteppers.jl:690-702 (string -> stepper; keywords go to the
+    filtered steppers' makefilter)."""
     try:
         cls = _STEPPERS[stepper]
     except KeyError:
-        raise ValueError(f"time stepper {stepper!r} is not part of this build "
-                         f"(available: {sorted(_STEPPERS)})") from None
-    return cls(equation, dt, dev)
+        raise ValueError(f"unknown time stepper {stepper!r} (available: {sorted(_STEPPERS)})") from None
+    return cls(equation, dt, dev, **kw)
 
 
 class EmptyParams:
@@ -217,7 +392,7 @@ class EmptyVars:
 class Problem:
     """``Problem(eqn, stepper, dt, grid, vars, params)`` (src/problem.jl:99-111)."""
 
-    def __init__(self, eqn, stepper, dt, grid, vars=None, params=None):
+    def __init__(self, eqn, stepper, dt, grid, vars=None, params=None, **stepperkw):
         dev = grid.device
         if not isinstance(dev, HIP):
             raise RuntimeError("libffhip problems run on HIP devices")
@@ -225,7 +400,7 @@ class Problem:
         self.vars = EmptyVars() if vars is None else vars
         self.params = EmptyParams() if params is None else params
         self.clock = Clock(float(np.dtype(fltype(grid.T)).type(dt)))
-        self.timestepper = TimeStepper(stepper, eqn, dt, dev)
+        self.timestepper = TimeStepper(stepper, eqn, dt, dev, **stepperkw)
         self.stepper = stepper
         self.sol = zeros(dev, eqn.T, eqn.dims)
 

@@ -155,6 +155,29 @@ int ffhip_rk4_substepsol(ffhip_rk4* ts, void* newsol, const void* sol, const voi
 int ffhip_rk4_update(ffhip_rk4* ts, void* sol, const void* R1, const void* R2, const void* R3,
                      const void* R4, double dt, void* stream);
 
+/* ---------------------------------------------------------------- filters and explicit steppers
+ * makefilter (src/domains.jl:506-543) is computed by the host; attaching it
+ * to a stepper makes it the Filtered* variant: sol *= filter after every
+ * step (src/timesteppers.jl:142-149, 276-285, 405-414, 549-558, 653-666).
+ * filter: host FP64, prod(dims) entries; NULL detaches.  The fused and
+ * generic drivers (ffhip_op_*_run) apply it. */
+int ffhip_etdrk4_set_filter(ffhip_etdrk4* ts, const double* filter);
+int ffhip_rk4_set_filter(ffhip_rk4* ts, const double* filter);
+int ffhip_etdrk4_apply_filter(ffhip_etdrk4* ts, void* sol, void* stream);
+int ffhip_rk4_apply_filter(ffhip_rk4* ts, void* sol, void* stream);
+/* ForwardEuler / AB3 / LSRK54 (src/timesteppers.jl:98-150, 318-414, 565-667)
+ * share the RK4 object (L, filter, scratch).
+ * euler_update (:112-118, :142-149): sol = [filter*](sol + dt (L sol + N)).
+ * ab3_update (AB3update!, :630-638): step < 3: sol += dt RHS, else
+ *   sol += dt (23/12 RHS - 16/12 RHSm1 + 5/12 RHSm2); then the filter.
+ * lsrk54_stage (:389-397): S2 = A S2 + dt RHS; sol += B S2 (A = 0 does not
+ *   read S2, which is what the reference's S2 = 0 start gives). */
+int ffhip_euler_update(ffhip_rk4* ts, void* sol, const void* N, double dt, void* stream);
+int ffhip_ab3_update(ffhip_rk4* ts, void* sol, const void* RHS, const void* RHSm1, const void* RHSm2, double dt,
+                     int64_t step, void* stream);
+int ffhip_lsrk54_stage(ffhip_rk4* ts, void* sol, void* S2, const void* RHS, double A, double B, double dt,
+                       void* stream);
+
 /* ---------------------------------------------------------------- reductions
  * parsevalsum2 / parsevalsum (src/utils.jl:113-183), compensated FP64.
  * rfft != 0: dims[0] == nkr and the kx = 0 and kx = nx/2 entries get weight 1,
@@ -193,6 +216,12 @@ int ffhip_op_calcN(ffhip_op* op, void* N, const void* sol, double t, void* strea
 int ffhip_op_etdrk4_run(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, void* stream);
 /* nsteps fused RK4 steps with the linear term of ts and step dt. */
 int ffhip_op_rk4_run(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t nsteps, void* stream);
+/* nsteps of ForwardEuler (kind 0), AB3 (1) or LSRK54 (2) with the built-in
+ * operator, filtered if ts has a filter; step0 = clock.step at entry (AB3
+ * takes forward Euler steps while step < 3).  AB3 reads and updates the
+ * caller's RHS_-1, RHS_-2 arrays (rhs_m1, rhs_m2; ignored otherwise). */
+int ffhip_op_explicit_run(ffhip_op* op, ffhip_rk4* ts, int kind, void* sol, double dt, int64_t nsteps,
+                          int64_t step0, void* rhs_m1, void* rhs_m2, void* stream);
 /* Diagnostics on the rfft state: out[0] = energy, out[1] = enstrophy
  * (vorticity: 1/2<|u|^2>, 1/2<zeta^2>; tracer: out[0] = 1/2<c^2>, out[1] = <c>). */
 int ffhip_op_diagnostics(ffhip_op* op, const void* sol, double* out, void* stream);

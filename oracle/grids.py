@@ -232,3 +232,24 @@ def kept_mask(grid, spectral_shape=None):
     m = np.ones(shape, dtype=bool)
     dealias(m, grid)
     return m
+
+
+def makefilter(grid, realvars=True, order=4, innerK=2 / 3, outerK=1, tol=1e-15, dims=None):
+    """src/domains.jl:506-543: exp(-decay (K - innerK)^order) with
+    decay = -log(tol)/(outerK - innerK)^order and 1 where K < innerK, K the
+    wavenumber magnitude scaled by dx/pi per axis (realvars: rfft layout).
+    ``dims`` (C order) broadcasts it to an equation's shape
+    (``makefilter(g, T, sz)``, :541)."""
+    ks = [(grid.kr if realvars else grid.k) * grid.dx / np.pi]
+    if grid.ndim > 1:
+        ks.append(grid.l * grid.dy / np.pi)
+    if grid.ndim > 2:
+        ks.append(grid.m * grid.dz / np.pi)
+    K = np.sqrt(sum(k**2 for k in ks)) if grid.ndim > 1 else np.abs(ks[0])
+    decay = -np.log(tol) / (outerK - innerK) ** order
+    filt = np.exp(-decay * (K - innerK) ** order)
+    filt = np.where(K < innerK, 1.0, filt)
+    if dims is not None:
+        filt = np.ones(dims) * filt
+    return filt
+

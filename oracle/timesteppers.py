@@ -130,25 +130,179 @@ class RK4TimeStepper:
         clock.step += 1
 
 
-_STEPPERS = {"ETDRK4": ETDRK4TimeStepper, "RK4": RK4TimeStepper}
+def _filter_for(eqn, filterkw):
+    """makefilter(equation; kw...) (src/domains.jl:542): realvars iff the
+    equation's leading Julia dimension is the grid's nkr."""
+    from .grids import makefilter
+
+    g = eqn.grid
+    realvars = eqn.dims[-1] == g.nkr
+    return makefilter(g, realvars=realvars, dims=eqn.dims, **filterkw)
 
 
-def TimeStepper(stepper, eqn, dt):
-    """src/timesteppers.jl:57-69 (string -> stepper)."""
+class FilteredETDRK4TimeStepper(ETDRK4TimeStepper):
+    """src/timesteppers.jl:471-499, 549-558: ETDRK4 then sol *= filter."""
+
+    def __init__(self, eqn, dt, **filterkw):
+        super().__init__(eqn, dt)
+        self.filter = _filter_for(eqn, filterkw)
+
+    def stepforward(self, sol, clock, eqn, vars, params, grid):
+        self.substeps(sol, clock, eqn, vars, params, grid)
+        sol[...] = (self.expLdt * sol + self.alpha * self.N1
+                    + 2 * self.beta * (self.N2 + self.N3) + self.gamma * self.N4)
+        sol *= self.filter
+        clock.t += clock.dt
+        clock.step += 1
+
+
+class FilteredRK4TimeStepper(RK4TimeStepper):
+    """src/timesteppers.jl:200-223, 276-285: RK4 then sol *= filter."""
+
+    def __init__(self, eqn, dt=None, **filterkw):
+        super().__init__(eqn, dt)
+        self.filter = _filter_for(eqn, filterkw)
+
+    def stepforward(self, sol, clock, eqn, vars, params, grid):
+        dt = clock.dt
+        self.substeps(sol, clock, eqn, vars, params, grid, clock.t, dt)
+        sol += dt / 6 * (self.RHS1 + 2 * self.RHS2 + 2 * self.RHS3 + self.RHS4)
+        sol *= self.filter
+        clock.t += clock.dt
+        clock.step += 1
+
+
+class ForwardEulerTimeStepper:
+    """src/timesteppers.jl:98-118: sol += dt (L sol + N)."""
+
+    def __init__(self, eqn, dt=None):
+        self.N = np.zeros(eqn.dims, dtype=eqn.T)
+        self.filter = None
+
+    def stepforward(self, sol, clock, eqn, vars, params, grid):
+        eqn.calcN(self.N, sol, clock.t, clock, vars, params, grid)
+        if self.filter is None:
+            sol += clock.dt * (eqn.L * sol + self.N)
+        else:  # :142-149
+            sol[...] = self.filter * (sol + clock.dt * (self.N + eqn.L * sol))
+        clock.t += clock.dt
+        clock.step += 1
+
+
+class FilteredForwardEulerTimeStepper(ForwardEulerTimeStepper):
+    """src/timesteppers.jl:126-149."""
+
+    def __init__(self, eqn, dt=None, **filterkw):
+        super().__init__(eqn, dt)
+        self.filter = _filter_for(eqn, filterkw)
+
+
+# src/timesteppers.jl:564-566
+AB3H1, AB3H2, AB3H3 = 23 / 12, 16 / 12, 5 / 12
+
+
+class AB3TimeStepper:
+    """src/timesteppers.jl:588-650: forward Euler for the first two steps, then
+    sol += dt (23/12 RHS - 16/12 RHS_-1 + 5/12 RHS_-2)."""
+
+    def __init__(self, eqn, dt=None):
+        z = lambda: np.zeros(eqn.dims, dtype=eqn.T)  # noqa: E731
+        self.RHS, self.RHSm1, self.RHSm2 = z(), z(), z()
+        self.filter = None
+
+    def stepforward(self, sol, clock, eqn, vars, params, grid):
+        eqn.calcN(self.RHS, sol, clock.t, clock, vars, params, grid)
+        self.RHS += eqn.L * sol
+        if clock.step < 3:  # AB3update! (:630-638)
+            sol += clock.dt * self.RHS
+        else:
+            sol += clock.dt * (AB3H1 * self.RHS - AB3H2 * self.RHSm1 + AB3H3 * self.RHSm2)
+        if self.filter is not None:
+            sol *= self.filter
+        clock.t += clock.dt
+        clock.step += 1
+        self.RHSm2[...] = self.RHSm1
+        self.RHSm1[...] = self.RHS
+
+
+class FilteredAB3TimeStepper(AB3TimeStepper):
+    """src/timesteppers.jl:609-628, 653-666."""
+
+    def __init__(self, eqn, dt=None, **filterkw):
+        super().__init__(eqn, dt)
+        self.filter = _filter_for(eqn, filterkw)
+
+
+# src/timesteppers.jl:330-345 (Carpenter & Kennedy 1994, 5-stage 4th order, 2N storage)
+LSRK54_A = (0.0, -567301805773 / 1357537059087, -2404267990393 / 2016746695238,
+            -3550918686646 / 2091501179385, -1275806237668 / 842570457699)
+LSRK54_B = (1432997174477 / 9575080441755, 5161836677717 / 13612068292357, 1720146321549 / 2090206949498,
+            3134564353537 / 4481467310338, 2277821191437 / 14882151754819)
+LSRK54_C = (0.0, 1432997174477 / 9575080441755, 2526269341429 / 6820363962896,
+            2006345519317 / 3224310063776, 2802321613138 / 2924317926251)
+
+
+class LSRK54TimeStepper:
+    """src/timesteppers.jl:318-414: for i = 1..5: RHS = N(sol, t + C_i dt) + L sol;
+    S2 = A_i S2 + dt RHS; sol += B_i S2."""
+
+    def __init__(self, eqn, dt=None):
+        self.S2 = np.zeros(eqn.dims, dtype=eqn.T)
+        self.RHS = np.zeros(eqn.dims, dtype=eqn.T)
+        self.filter = None
+
+    def stepforward(self, sol, clock, eqn, vars, params, grid):
+        t, dt = clock.t, clock.dt
+        self.S2[...] = 0
+        for i in range(5):
+            eqn.calcN(self.RHS, sol, t + LSRK54_C[i] * dt, clock, vars, params, grid)
+            self.RHS += eqn.L * sol
+            self.S2[...] = LSRK54_A[i] * self.S2 + dt * self.RHS
+            sol += LSRK54_B[i] * self.S2
+        if self.filter is not None:
+            sol *= self.filter
+        clock.t += clock.dt
+        clock.step += 1
+
+
+class FilteredLSRK54TimeStepper(LSRK54TimeStepper):
+    """src/timesteppers.jl:365-387, 405-414."""
+
+    def __init__(self, eqn, dt=None, **filterkw):
+        super().__init__(eqn, dt)
+        self.filter = _filter_for(eqn, filterkw)
+
+
+_STEPPERS = {"ETDRK4": ETDRK4TimeStepper, "RK4": RK4TimeStepper, "ForwardEuler": ForwardEulerTimeStepper,
+             "AB3": AB3TimeStepper, "LSRK54": LSRK54TimeStepper, "FilteredETDRK4": FilteredETDRK4TimeStepper,
+             "FilteredRK4": FilteredRK4TimeStepper, "FilteredForwardEuler": FilteredForwardEulerTimeStepper,
+             "FilteredAB3": FilteredAB3TimeStepper, "FilteredLSRK54": FilteredLSRK54TimeStepper}
+
+# src/timesteppers.jl:672-681
+FULLY_EXPLICIT = ("ForwardEuler", "RK4", "AB3", "LSRK54", "FilteredForwardEuler", "FilteredRK4", "FilteredAB3",
+                  "FilteredLSRK54")
+
+
+def isexplicit(stepper):
+    return stepper in FULLY_EXPLICIT
+
+
+def TimeStepper(stepper, eqn, dt, **kw):
+    """src/timesteppers.jl:690-702 (string -> stepper; keywords go to the filter)."""
     try:
         cls = _STEPPERS[stepper]
     except KeyError:
         raise ValueError(f"unknown time stepper {stepper!r}") from None
-    return cls(eqn, dt)
+    return cls(eqn, dt, **kw)
 
 
 class Problem:
     """src/problem.jl:99-111."""
 
-    def __init__(self, eqn, stepper, dt, grid, vars=None, params=None):
+    def __init__(self, eqn, stepper, dt, grid, vars=None, params=None, **stepperkw):
         self.eqn, self.grid, self.vars, self.params = eqn, grid, vars, params
         self.clock = Clock(dt)
-        self.timestepper = TimeStepper(stepper, eqn, dt)
+        self.timestepper = TimeStepper(stepper, eqn, dt, **stepperkw)
         self.sol = np.zeros(eqn.dims, dtype=eqn.T)
 
     def stepforward(self, nsteps=1):
@@ -160,7 +314,7 @@ class Problem:
 
 def step_until(prob, stop_time):
     """src/timesteppers.jl:734-760 (rejects ETDRK4 at :736-737)."""
-    if isinstance(prob.timestepper, ETDRK4TimeStepper):
+    if isinstance(prob.timestepper, (ETDRK4TimeStepper,)):
         raise RuntimeError("step_until! requires fully explicit time stepper; does not work with ETDRK4")
     if not stop_time > prob.clock.t:
         raise ValueError("stop_time must be greater than prob.clock.t")

@@ -48,12 +48,13 @@ def _oracle(n, ny, Ly, zh0, L, dt, nsteps):
 
 
 @pytest.mark.parametrize("n,ny,Ly", [(64, 64, 2 * np.pi), (128, 128, 2 * np.pi), (256, 128, 4 * np.pi),
-                                     (128, 512, 2 * np.pi), (512, 512, 2 * np.pi)])
+                                     (128, 512, 2 * np.pi), (512, 512, 2 * np.pi), (1024, 1024, 2 * np.pi),
+                                     (2048, 1024, np.pi)])
 def test_fused_fp64_matches_oracle(ff, n, ny, Ly):
     o = og.TwoDGrid(n, 2 * np.pi, ny, Ly, x0=0, y0=0)
     zh0 = oics.vorticity_ic(o, seed=11, k0=min(6.0, n / 10))
     L = oph.vorticity_linear(o, nu=2e-3)
-    steps = 30
+    steps = 30 if n * ny <= 512 * 512 else 6  # >= 1024: the three-team FP64 passes
     ref = _oracle(n, ny, Ly, zh0, L, 0.004, steps)
     prob = ff.TwoDVorticityProblem(nx=n, ny=ny, Ly=Ly, nu=2e-3, dt=0.004)
     assert prob.eqn.calcN.fused

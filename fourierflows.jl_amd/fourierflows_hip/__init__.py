@@ -6,9 +6,10 @@ include/ffhip.h) that keeps the reference's API names:
 * grids: ``OneDGrid``, ``TwoDGrid``, ``ThreeDGrid`` with ``rfftplan`` /
   ``fftplan``, ``mul`` (``mul!``), ``ldiv`` (``ldiv!``), ``dealias``
   (``dealias!``), ``getaliasedwavenumbers``        (src/domains.jl)
-* ``Equation``, ``Clock``, ``Problem``, ``TimeStepper``,
-  ``ETDRK4TimeStepper``, ``RK4TimeStepper``, ``stepforward``
-  (``stepforward!``), ``step_until``       (src/problem.jl, src/timesteppers.jl)
+* ``Equation``, ``Clock``, ``Problem``, ``TimeStepper``, the ETDRK4, RK4,
+  ForwardEuler, AB3 and LSRK54 steppers and their Filtered variants
+  (``makefilter``), ``stepforward`` (``stepforward!``), ``step_until``
+                                           (src/problem.jl, src/timesteppers.jl)
 * ``parsevalsum2``, ``parsevalsum``               (src/utils.jl)
 * built-in operators ``VorticityOperator`` / ``TracerOperator`` and the
   ``Diffusion`` testbed                              (src/diffusion.jl)
@@ -23,12 +24,15 @@ from . import _lib, adjoint
 from ._lib import DomainError, FFHIPError, NonFiniteError
 from .core import CPU, GPU, HIP, cxtype, device_array, fltype, init, zeros
 from .domains import (OneDGrid, Plan, ThreeDGrid, TwoDGrid, dealias, getaliasedwavenumbers, gridpoints, ldiv,
-                      mul)
+                      makefilter, mul)
 from .ics import ensemble_members, splitmix64, tracer_ic, uniform01, vorticity_ic
 from .physics import (Diffusion, TracerOperator, TracerProblem, TwoDVorticityProblem, VorticityEnsemble,
                       VorticityOperator, vorticity_linear)
-from .timesteppers import (Clock, EmptyParams, EmptyVars, Equation, ETDRK4TimeStepper, Problem, RK4TimeStepper,
-                           TimeStepper, isexplicit, step_until, stepforward)
+from .timesteppers import (AB3TimeStepper, Clock, EmptyParams, EmptyVars, Equation, ETDRK4TimeStepper,
+                           FilteredAB3TimeStepper, FilteredETDRK4TimeStepper, FilteredForwardEulerTimeStepper,
+                           FilteredLSRK54TimeStepper, FilteredRK4TimeStepper, ForwardEulerTimeStepper,
+                           LSRK54TimeStepper, Problem, RK4TimeStepper, TimeStepper, isexplicit, step_until,
+                           stepforward)
 from .utils import etdcoeffs, parsevalsum, parsevalsum2
 
 # Julia-style aliases

@@ -268,6 +268,38 @@ def gridpoints(grid):
     return X, Y, Z
 
 
+def makefilter(grid, realvars=True, order=4, innerK=2 / 3, outerK=1, tol=1e-15, dims=None):
+    """src/domains.jl:506-543 (host FP64 array, C order).
+
+    ``K`` is the wavenumber magnitude with each axis scaled by d/pi, computed
+    in the grid's float type as the reference does; the filter is
+    ``exp(-decay (K - innerK)^order)`` with ``decay = -log(tol)/(outerK -
+    innerK)^order``, and 1 where ``K < innerK``.  ``dims`` (Julia order)
+    broadcasts it to an equation's shape (``makefilter(g, T, sz)``, :541).
+    """
+    T = np.dtype(grid.T).type
+    pi = T(np.pi)
+    ks = [np.asarray(grid.kr_host if realvars else grid.k_host, dtype=T) * T(grid.dx) / pi]
+    if grid.ndim > 1:
+        ks.append(np.asarray(grid.l_host, dtype=T) * T(grid.dy) / pi)
+    if grid.ndim > 2:
+        ks.append(np.asarray(grid.m_host, dtype=T) * T(grid.dz) / pi)
+    if grid.ndim == 1:
+        K = np.abs(ks[0])
+    else:
+        acc = ks[0] ** 2
+        for k in ks[1:]:
+            acc = acc + k**2
+        K = np.sqrt(acc)
+    K = K.astype(np.float64)
+    decay = -np.log(tol) / (outerK - innerK) ** order
+    f = np.exp(-decay * (K - innerK) ** order)
+    f = np.where(K < innerK, 1.0, f)
+    if dims is not None:
+        f = np.ones(tuple(reversed(tuple(dims)))) * f
+    return np.ascontiguousarray(f)
+
+
 def dealias(fh, grid):
     """``dealias!(fh, grid)`` on a device array (src/domains.jl:428-476).
 
@@ -289,5 +321,3 @@ def dealias(fh, grid):
     return None
 
 
-def makefilter(*args, **kwargs):  # pragma: no cover - out of scope for the hot path
-    raise NotImplementedError("spectral filters (Filtered* steppers) are not part of this build")

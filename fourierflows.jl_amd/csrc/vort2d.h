@@ -79,6 +79,7 @@ struct VortState {
   // the checkpointed forward run the checkpoints belong to
   const void* ck_ts = nullptr;
   int64_t ck_nsteps = -1, ck_ncheck = 0;
+  bool fs_last = false;  // FS holds the last segment's fields from the forward run
 };
 
 __device__ __forceinline__ int vfreq(int i, int n) { return i < n / 2 ? i : i - n; }

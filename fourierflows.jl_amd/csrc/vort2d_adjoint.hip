@@ -478,11 +478,14 @@ static int vforward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void*
   const size_t fb = vfield_bytes<R>(g);
   FF_TRY(vort_import(prec, g, sol0, s->U, st));
   FF_TRY(vort_col(prec, 0, g, ca, st));
+  s->fs_last = false;
   for (int64_t k0 = 0, c = 0; k0 < nsteps; k0 += seg, ++c) {
     FF_HIP(hipMemcpyAsync((char*)s->CK + c * sb, s->U, sb, hipMemcpyDeviceToDevice, st));
     const int64_t k1 = k0 + seg < nsteps ? k0 + seg : nsteps;
-    FF_TRY(fwd_steps(prec, s, ca, ra, k0, k1, k1 < nsteps, false, fb, st));
+    // the last segment keeps its stage fields, so the adjoint sweep starts without a recompute
+    FF_TRY(fwd_steps(prec, s, ca, ra, k0, k1, k1 < nsteps, k1 == nsteps, fb, st));
   }
+  s->fs_last = true;
   if (solT != sol0)
     FF_HIP(hipMemcpyAsync(solT, sol0, (size_t)g.ny * g.nkr * sizeof(cplx<R>), hipMemcpyDeviceToDevice, st));
   FF_TRY(vort_export(prec, g, solT, s->U, reinterpret_cast<const double*>(ts->coef[4]),
@@ -547,11 +550,14 @@ static int vadjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, void* lam0
   for (int64_t c = nseg - 1; c >= 0; --c) {
     const int64_t k0 = c * seg;
     const int64_t k1 = k0 + seg < nsteps ? k0 + seg : nsteps;
-    // recompute the segment, keeping every stage's fields
-    FF_HIP(hipMemcpyAsync(s->U, (char*)s->CK + c * sb, sb, hipMemcpyDeviceToDevice, st));
-    ca.do_inverse = 1;
-    FF_TRY(vort_col(prec, 0, g, ca, st));
-    FF_TRY(fwd_steps(prec, s, ca, ra, k0, k1, false, true, fb, st));
+    // recompute the segment, keeping every stage's fields (the forward run kept the last one's)
+    if (!(c == nseg - 1 && s->fs_last)) {
+      FF_HIP(hipMemcpyAsync(s->U, (char*)s->CK + c * sb, sb, hipMemcpyDeviceToDevice, st));
+      ca.do_inverse = 1;
+      FF_TRY(vort_col(prec, 0, g, ca, st));
+      FF_TRY(fwd_steps(prec, s, ca, ra, k0, k1, false, true, fb, st));
+    }
+    s->fs_last = false;
     // reverse sweep over the segment
     for (int64_t k = k1 - 1; k >= k0; --k) {
       const char* Fk = (const char*)s->FS + (k - k0) * 4 * fb;

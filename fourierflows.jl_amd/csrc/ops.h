@@ -23,6 +23,7 @@ struct ffhip_op {
   void* phys[4] = {nullptr, nullptr, nullptr, nullptr};
   void* tables = nullptr;  // tracer: sin/cos of the grid coordinates (FP64)
   void* fused = nullptr;   // fused implementation state (power-of-two vorticity grids)
+  void* fused3 = nullptr;  // fused RK4 state of the tracer (power-of-two 3D grids)
 };
 
 namespace ffhip {
@@ -32,6 +33,10 @@ int op_calcN(ffhip_op* op, void* N, const void* sol, hipStream_t st);
 int fused_create(ffhip_op* op);
 void fused_destroy(ffhip_op* op);
 int fused_etdrk4_run(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, hipStream_t st);
+// fused tracer RK4 (tracer3d_fused.hip)
+int tracer_fused_create(ffhip_op* op);
+void tracer_fused_destroy(ffhip_op* op);
+int tracer_fused_rk4_run(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t nsteps, hipStream_t st);
 // discrete adjoint (vort2d_adjoint.hip)
 int fused_forward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void* solT, int64_t nsteps,
                        int64_t ncheck, hipStream_t st);

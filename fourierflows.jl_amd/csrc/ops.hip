@@ -515,6 +515,7 @@ static int rk4_adjoint_tracer(ffhip_op* op, ffhip_rk4* ts, void* lam, double dt,
 static void op_free(ffhip_op* op) {
   if (!op) return;
   fused_destroy(op);
+  tracer_fused_destroy(op);
   if (op->plan) ffhip_plan_destroy(op->plan);
   for (auto& p : op->spec) dev_free(p);
   for (auto& p : op->phys) dev_free(p);
@@ -606,6 +607,7 @@ int ffhip_op_create(ffhip_op** out, int kind, int prec, int ndim, const int64_t*
     }
   }
   if (rc == FFHIP_OK) rc = fused_create(op);
+  if (rc == FFHIP_OK) rc = tracer_fused_create(op);
   if (rc != FFHIP_OK) {
     op_free(op);
     return rc;
@@ -647,6 +649,8 @@ int ffhip_op_rk4_run(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t 
   FF_CHECK(same_layout(op, ts->ndim, ts->dims, ts->prec), FFHIP_ERR_ARG,
            "stepper shape/precision does not match the operator's rfft layout");
   const double h = op->prec == FFHIP_F32 ? (double)(float)dt : dt;
+  if (nsteps == 0) return FFHIP_OK;
+  if (op->fused3 && !ts->cplx_coef && !ts->filter) return tracer_fused_rk4_run(op, ts, sol, h, nsteps, as_stream(stream));
   return rk4_run_generic(op, ts, sol, h, nsteps, as_stream(stream));
 }
 
@@ -711,7 +715,7 @@ int ffhip_op_explicit_run(ffhip_op* op, ffhip_rk4* ts, int kind, void* sol, doub
 int ffhip_op_is_fused(const ffhip_op* op, int* fused) {
   clear_error();
   FF_CHECK(op && fused, FFHIP_ERR_ARG, "null argument");
-  *fused = op->fused != nullptr;
+  *fused = op->fused != nullptr || op->fused3 != nullptr;
   return FFHIP_OK;
 }
 

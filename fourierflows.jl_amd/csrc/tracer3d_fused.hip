@@ -1,0 +1,644 @@
+// Fused RK4 step of the 3D passive tracer in the steady ABC flow on
+// power-of-two grids (64..512 per axis).
+//
+// One RK4 stage is four kernels over the kept (dealiased) band only:
+//   y-inverse   : inverse y-FFT of the z-inverted stage state        (per kx, z)
+//   x pass      : c2r in x, products u c, v c, w c with the analytic
+//                 velocity, forward x-FFTs of (u c + i v c) and w c   (per y, z row)
+//   y-forward   : forward y-FFTs of the three products, kept y only (per kx, z)
+//   z pass      : forward z-FFTs of the three products -> N = -i (kx U + l V + m W)
+//                 at the kept modes -> the RK4 stage of
+//                 src/timesteppers.jl:237-274 (addlinearterm!, substepsol!,
+//                 RK4update!, FP64 arithmetic, the state type at every store
+//                 as there) -> inverse z-FFT of the next stage state (per kx, y)
+// The reference's calcN! would run one c2r and three r2c 3D transforms plus
+// six broadcasts per stage over the full arrays.
+//
+// Layouts (complex, blocks of 4 kx as in the 2D passes):
+//   state, stage state, RHS1..3 : [TX][KY][KZ][4]
+//   y-pass input/output A1, A2  : [nz][TX][ny][4]   (A1's aliased y rows stay zero)
+//   x-pass output B (3 fields)  : [nz][TX][ny][4]
+//   z-pass input C (3 fields)   : [TX][KY][nz][4]
+#include <cstdlib>
+#include <new>
+
+#include "vort2d.h"
+
+namespace ffhip {
+
+struct TracerGeom {
+  int nx, ny, nz, nkr;
+  int KX, KXp, TX;
+  int ylo, yhi, KY;  // aliased y rows [ylo, yhi] (0-based), kept count
+  int zlo, zhi, KZ;
+  double dk, dl, dm;
+  double norm;       // 1 / (nx ny nz): the three unnormalised inverses
+  double A, B, C;    // ABC amplitudes
+};
+
+struct TracerState {
+  TracerGeom g;
+  void* S = nullptr;   // state
+  void* S1 = nullptr;  // stage state
+  void* R[3] = {nullptr, nullptr, nullptr};  // RHS1..3
+  void* A1 = nullptr;
+  void* A2 = nullptr;
+  void* Bf[3] = {nullptr, nullptr, nullptr};
+  void* Cf[3] = {nullptr, nullptr, nullptr};
+  void* Lp = nullptr;  // L packed to the state blocking (FP64)
+  const ffhip_rk4* lp_owner = nullptr;
+  void* twx = nullptr;
+  void* twy = nullptr;
+  void* twz = nullptr;
+  const double* tab = nullptr;  // sin x, cos x | sin y, cos y | sin z, cos z (FP64, owned by the operator)
+};
+
+struct TracerArgs {
+  void* S;
+  void* S1;
+  void* R0;
+  void* R1;
+  void* R2;
+  void* A1;
+  void* A2;
+  void* B0;
+  void* B1;
+  void* B2;
+  void* C0;
+  void* C1;
+  void* C2;
+  const double* L;
+  const double* tab;
+  const void* twx;
+  const void* twy;
+  const void* twz;
+  double h2, h6, h;  // dt/2 (rounded as the reference rounds it), dt/6, dt
+  int do_inverse;
+};
+
+__device__ __forceinline__ int t_kept_index(int i, int lo, int hi) { return i < lo ? i : i - (hi - lo + 1); }
+__device__ __forceinline__ int t_full_index(int k, int lo, int hi) { return k < lo ? k : k + (hi - lo + 1); }
+
+// ----------------------------------------------------------------- z pass
+// STAGE 0: inverse z-FFT of the state only.  STAGE 1..4: RK4 stages.
+template <typename R, int NZ, int P, int STAGE>
+__global__ void __launch_bounds__(kVCB*(NZ / P)) k_tz(TracerGeom g, TracerArgs a) {
+  constexpr int T = NZ / P;
+  constexpr int CB = kVCB;
+  extern __shared__ unsigned char tsm[];
+  const int grp = (int)blockIdx.x;  // one 4-column tile at one kept y per workgroup
+  const int tx = grp / g.KY, ky = grp % g.KY;
+  const int t = threadIdx.x;
+  const int c = t % CB;
+  const int tid = t / CB;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * TeamSched<NZ, P>::LDS;
+  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.twz);
+  const int ix = tx * CB + c;
+  const bool colkept = ix < g.KX;
+  const int iy = t_full_index(ky, g.ylo, g.yhi);
+  const R kx = (R)(g.dk * (double)ix);
+  const R l = (R)(g.dl * (double)vfreq(iy, g.ny));
+  const int64_t cbase = ((int64_t)tx * g.KY + ky) * NZ * CB + c;      // C[f][tx][ky][z][c]
+  const int64_t sbase = ((int64_t)tx * g.KY + ky) * g.KZ * CB + c;    // state [tx][ky][kz][c]
+
+  cplx<R> nn[P];
+  if constexpr (STAGE > 0) {
+    // N = -i (kx U + l V + m W), accumulated field by field in the state type
+    const cplx<R>* Cs[3] = {reinterpret_cast<const cplx<R>*>(a.C0), reinterpret_cast<const cplx<R>*>(a.C1),
+                            reinterpret_cast<const cplx<R>*>(a.C2)};
+#pragma unroll
+    for (int p = 0; p < P; ++p) nn[p] = mkc<R>(0, 0);
+#pragma unroll 1
+    for (int f = 0; f < 3; ++f) {
+      cplx<R> v[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) v[p] = Cs[f][cbase + (int64_t)(tid + p * T) * CB];
+      team_fft<NZ, P, -1, R>(v, lds, tid, tw);
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const int iz = tid + p * T;
+        const R q = f == 0 ? kx : (f == 1 ? l : (R)(g.dm * (double)vfreq(iz, NZ)));
+        // running sum s = kx U + l V + m W in the order of k_tracer_assemble
+        if (f == 0) {
+          nn[p] = mkc<R>(q * v[p].x, q * v[p].y);
+        } else {
+          nn[p] = mkc<R>(nn[p].x + q * v[p].x, nn[p].y + q * v[p].y);
+        }
+      }
+    }
+  }
+
+  cplx<R>* S = reinterpret_cast<cplx<R>*>(a.S);
+  cplx<R>* S1 = reinterpret_cast<cplx<R>*>(a.S1);
+  cplx<R>* R0 = reinterpret_cast<cplx<R>*>(a.R0);
+  cplx<R>* R1 = reinterpret_cast<cplx<R>*>(a.R1);
+  cplx<R>* R2 = reinterpret_cast<cplx<R>*>(a.R2);
+  cplx<R> nw[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int iz = tid + p * T;
+    nw[p] = mkc<R>(0, 0);
+    const bool kept = colkept && (iz < g.zlo || iz > g.zhi);
+    if (!kept) continue;
+    const int64_t si = sbase + (int64_t)t_kept_index(iz, g.zlo, g.zhi) * CB;
+    if constexpr (STAGE == 0) {
+      nw[p] = S[si];
+    } else {
+      // calcN! output in the state type (N = -i s), then addlinearterm!: RHS = N + L x
+      const cplx<R> N = mkc<R>(nn[p].y, -nn[p].x);
+      const cplx<R> x = STAGE == 1 ? S[si] : S1[si];
+      const double L = a.L[si];
+      const cplx<R> rhs = mkc<R>((R)((double)N.x + L * (double)x.x), (R)((double)N.y + L * (double)x.y));
+      const cplx<R> u = STAGE == 1 ? x : S[si];
+      if constexpr (STAGE < 4) {
+        // substepsol!: sol + h RHS (h = dt/2, dt/2, dt)
+        const double h = STAGE == 3 ? a.h : a.h2;
+        (STAGE == 1 ? R0 : (STAGE == 2 ? R1 : R2))[si] = rhs;
+        nw[p] = mkc<R>((R)((double)u.x + h * (double)rhs.x), (R)((double)u.y + h * (double)rhs.y));
+        S1[si] = nw[p];
+      } else {
+        // RK4update!: sol += dt/6 (RHS1 + 2 RHS2 + 2 RHS3 + RHS4)
+        const cplx<R> r1 = R0[si], r2 = R1[si], r3 = R2[si];
+        double sx = (double)r1.x + 2.0 * (double)r2.x, sy = (double)r1.y + 2.0 * (double)r2.y;
+        sx = sx + 2.0 * (double)r3.x;
+        sy = sy + 2.0 * (double)r3.y;
+        sx = sx + (double)rhs.x;
+        sy = sy + (double)rhs.y;
+        nw[p] = mkc<R>((R)((double)u.x + a.h6 * sx), (R)((double)u.y + a.h6 * sy));
+        S[si] = nw[p];
+      }
+    }
+  }
+  if (!a.do_inverse) return;  // uniform over the block
+  team_fft<NZ, P, 1, R>(nw, lds, tid, reinterpret_cast<const cplx<R>*>(a.twz));
+  cplx<R>* A1 = reinterpret_cast<cplx<R>*>(a.A1);
+  const int64_t plane = (int64_t)g.TX * g.ny * CB;  // one z plane of A1
+  const int64_t abase = ((int64_t)tx * g.ny + iy) * CB + c;
+#pragma unroll
+  for (int p = 0; p < P; ++p) A1[(int64_t)(tid + p * T) * plane + abase] = nw[p];
+}
+
+// ----------------------------------------------------------------- y passes
+// DIR = +1: inverse y-FFT of A1 (kept rows) into A2.  DIR = -1: forward y-FFTs
+// of the three products B into C (kept rows only).
+template <typename R, int NY, int P, int DIR>
+__global__ void __launch_bounds__(kVCB*(NY / P)) k_ty(TracerGeom g, TracerArgs a) {
+  constexpr int T = NY / P;
+  constexpr int CB = kVCB;
+  extern __shared__ unsigned char tsm[];
+  const int grp = (int)blockIdx.x;  // one 4-column tile at one z per workgroup
+  const int iz = grp / g.TX, tx = grp % g.TX;
+  const int t = threadIdx.x;
+  const int c = t % CB;
+  const int tid = t / CB;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * TeamSched<NY, P>::LDS;
+  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.twy);
+  const int64_t base = ((int64_t)iz * g.TX + tx) * NY * CB + c;  // [z][tx][y][c]
+  if constexpr (DIR == 1) {
+    const cplx<R>* A1 = reinterpret_cast<const cplx<R>*>(a.A1);
+    cplx<R>* A2 = reinterpret_cast<cplx<R>*>(a.A2);
+    cplx<R> v[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int iy = tid + p * T;
+      v[p] = (iy < g.ylo || iy > g.yhi) ? A1[base + (int64_t)iy * CB] : mkc<R>(0, 0);
+    }
+    team_fft<NY, P, 1, R>(v, lds, tid, tw);
+#pragma unroll
+    for (int p = 0; p < P; ++p) A2[base + (int64_t)(tid + p * T) * CB] = v[p];
+  } else {
+    const cplx<R>* Bs[3] = {reinterpret_cast<const cplx<R>*>(a.B0), reinterpret_cast<const cplx<R>*>(a.B1),
+                            reinterpret_cast<const cplx<R>*>(a.B2)};
+    cplx<R>* Cs[3] = {reinterpret_cast<cplx<R>*>(a.C0), reinterpret_cast<cplx<R>*>(a.C1),
+                      reinterpret_cast<cplx<R>*>(a.C2)};
+#pragma unroll 1
+    for (int f = 0; f < 3; ++f) {
+      cplx<R> v[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) v[p] = Bs[f][base + (int64_t)(tid + p * T) * CB];
+      team_fft<NY, P, -1, R>(v, lds, tid, tw);
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const int iy = tid + p * T;
+        if (iy >= g.ylo && iy <= g.yhi) continue;
+        const int ky = t_kept_index(iy, g.ylo, g.yhi);
+        Cs[f][(((int64_t)tx * g.KY + ky) * g.nz + iz) * CB + c] = v[p];
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------- x pass
+template <typename R, int NX, int P>
+__global__ void __launch_bounds__(4 * (NX / P)) k_tx(TracerGeom g, TracerArgs a) {
+  constexpr int T = NX / P;
+  constexpr int CB = kVCB;
+  extern __shared__ unsigned char tsm[];
+  const int grp = (int)blockIdx.x;  // four rows (one 128 B line group) at one z per workgroup
+  const int iz = grp / (g.ny / 4);
+  const int r = (int)(threadIdx.x % 4);
+  const int y = (grp % (g.ny / 4)) * 4 + r;
+  const int tid = threadIdx.x / 4;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + r * TeamSched<NX, P>::LDS;
+  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.twx);
+  const int KX = g.KX;
+  const int64_t ny = g.ny;
+  const int64_t zoff = (int64_t)iz * g.TX * ny * CB;
+  auto xaddr = [&](int k) -> int64_t { return zoff + ((int64_t)(k / CB) * ny + y) * CB + (k % CB); };
+  const cplx<R>* A2 = reinterpret_cast<const cplx<R>*>(a.A2);
+
+  // c (real) as one complex inverse of its Hermitian-extended half spectrum
+  cplx<R> w[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int k = tid + p * T;
+    cplx<R> s = mkc<R>(0, 0);
+    if (k == 0) {
+      s = mkc<R>(A2[xaddr(0)].x, 0);  // c2r ignores Im at kx = 0
+    } else if (k < KX) {
+      s = A2[xaddr(k)];
+    } else if (k > NX - KX) {
+      s = cconj<R>(A2[xaddr(NX - k)]);
+    }
+    w[p] = s;
+  }
+  team_fft<NX, P, 1, R>(w, lds, tid, tw);
+
+  // u = A sin z + C cos y (constant along the row), v = B sin x + A cos z, w = C sin y + B cos x
+  const double* sx = a.tab;
+  const double* cx = a.tab + g.nx;
+  const double* sy = a.tab + 2 * g.nx;
+  const double* cy = sy + g.ny;
+  const double* sz = a.tab + 2 * g.nx + 2 * g.ny;
+  const double* cz = sz + g.nz;
+  const R u = (R)(g.A * sz[iz] + g.C * cy[y]);
+  const double acz = g.A * cz[iz], csy = g.C * sy[y];
+  const R nrm = (R)g.norm;
+  cplx<R> z[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int ix = tid + p * T;
+    const R cval = w[p].x * nrm;
+    const R v = (R)(g.B * sx[ix] + acz);
+    const R ww = (R)(csy + g.B * cx[ix]);
+    z[p] = mkc<R>(u * cval, v * cval);  // u c + i v c
+    w[p] = mkc<R>(ww * cval, 0);        // w c
+  }
+  team_fft<NX, P, -1, R>(z, lds, tid, tw);
+  // split (u c + i v c) into the half spectra of u c and v c
+#pragma unroll
+  for (int p = 0; p < P; ++p) lds[lpad(tid + p * T)] = z[p];
+  __syncthreads();
+  cplx<R>* B0 = reinterpret_cast<cplx<R>*>(a.B0);
+  cplx<R>* B1 = reinterpret_cast<cplx<R>*>(a.B1);
+  cplx<R>* B2 = reinterpret_cast<cplx<R>*>(a.B2);
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int k = tid + p * T;
+    if (k >= g.KXp) continue;
+    cplx<R> Pk = mkc<R>(0, 0), Qk = mkc<R>(0, 0);
+    if (k < KX) {
+      const cplx<R> wk = z[p];
+      const cplx<R> wc = cconj<R>(lds[lpad((NX - k) & (NX - 1))]);
+      Pk = mkc<R>(R(0.5) * (wk.x + wc.x), R(0.5) * (wk.y + wc.y));
+      Qk = mkc<R>(R(0.5) * (wk.y - wc.y), -R(0.5) * (wk.x - wc.x));
+    }
+    B0[xaddr(k)] = Pk;
+    B1[xaddr(k)] = Qk;
+  }
+  __syncthreads();
+  team_fft<NX, P, -1, R>(w, lds, tid, tw);
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int k = tid + p * T;
+    if (k >= g.KXp) continue;
+    B2[xaddr(k)] = k < KX ? w[p] : mkc<R>(0, 0);
+  }
+}
+
+// ----------------------------------------------------------------- layout conversion
+template <typename R>
+__global__ void k_timport(const cplx<R>* __restrict__ sol, cplx<R>* __restrict__ S, TracerGeom g) {
+  const int64_t total = (int64_t)g.TX * g.KY * g.KZ * kVCB;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % kVCB);
+    int64_t r = idx / kVCB;
+    const int kz = (int)(r % g.KZ);
+    r /= g.KZ;
+    const int ky = (int)(r % g.KY);
+    const int tx = (int)(r / g.KY);
+    const int ix = tx * kVCB + c;
+    const int iy = t_full_index(ky, g.ylo, g.yhi), iz = t_full_index(kz, g.zlo, g.zhi);
+    S[idx] = ix < g.KX ? sol[((int64_t)iz * g.ny + iy) * g.nkr + ix] : mkc<R>(0, 0);
+  }
+}
+
+static __global__ void k_tpackL(const double* __restrict__ L, double* __restrict__ Lp, TracerGeom g) {
+  const int64_t total = (int64_t)g.TX * g.KY * g.KZ * kVCB;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % kVCB);
+    int64_t r = idx / kVCB;
+    const int kz = (int)(r % g.KZ);
+    r /= g.KZ;
+    const int ky = (int)(r % g.KY);
+    const int tx = (int)(r / g.KY);
+    const int ix = tx * kVCB + c;
+    const int iy = t_full_index(ky, g.ylo, g.yhi), iz = t_full_index(kz, g.zlo, g.zhi);
+    Lp[idx] = ix < g.KX ? L[((int64_t)iz * g.ny + iy) * g.nkr + ix] : 0.0;
+  }
+}
+
+// Scatter the kept band back; modes outside it (N = 0 there) take the RK4
+// steps of du/dt = L u with the reference's substep roundings.
+template <typename R>
+__global__ void k_texport(cplx<R>* __restrict__ sol, const cplx<R>* __restrict__ S, const double* __restrict__ L,
+                          double h2, double h, double h6, int64_t nsteps, TracerGeom g) {
+  const int64_t total = (int64_t)g.nz * g.ny * g.nkr;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(idx % g.nkr);
+    const int64_t r = idx / g.nkr;
+    const int iy = (int)(r % g.ny), iz = (int)(r / g.ny);
+    const bool kept = ix < g.KX && (iy < g.ylo || iy > g.yhi) && (iz < g.zlo || iz > g.zhi);
+    if (kept) {
+      const int tx = ix / kVCB, c = ix % kVCB;
+      const int ky = t_kept_index(iy, g.ylo, g.yhi), kz = t_kept_index(iz, g.zlo, g.zhi);
+      sol[idx] = S[(((int64_t)tx * g.KY + ky) * g.KZ + kz) * kVCB + c];
+    } else {
+      cplx<R> u = sol[idx];
+      if (u.x == R(0) && u.y == R(0)) continue;
+      const double l = L[idx];
+      for (int64_t s = 0; s < nsteps; ++s) {
+        auto rhs = [&](cplx<R> x) { return mkc<R>((R)(l * (double)x.x), (R)(l * (double)x.y)); };
+        auto sub = [&](cplx<R> b, cplx<R> k, double hh) {
+          return mkc<R>((R)((double)b.x + hh * (double)k.x), (R)((double)b.y + hh * (double)k.y));
+        };
+        const cplx<R> r1 = rhs(u);
+        const cplx<R> r2 = rhs(sub(u, r1, h2));
+        const cplx<R> r3 = rhs(sub(u, r2, h2));
+        const cplx<R> r4 = rhs(sub(u, r3, h));
+        double sx = (double)r1.x + 2.0 * (double)r2.x, sy = (double)r1.y + 2.0 * (double)r2.y;
+        sx = sx + 2.0 * (double)r3.x;
+        sy = sy + 2.0 * (double)r3.y;
+        sx = sx + (double)r4.x;
+        sy = sy + (double)r4.y;
+        u = mkc<R>((R)((double)u.x + h6 * sx), (R)((double)u.y + h6 * sy));
+      }
+      sol[idx] = u;
+    }
+  }
+}
+
+// ----------------------------------------------------------------- host side
+static bool tpow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+template <typename R, int N>
+static size_t t_lds(int teams) {
+  return (size_t)teams * TeamSched<N, vpts_col(N)>::LDS * sizeof(cplx<R>);
+}
+
+template <typename K>
+static int t_attr(K kern, size_t bytes) {
+  FF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes));
+  return FFHIP_OK;
+}
+
+template <typename R, int N, int STAGE>
+static int launch_tz(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
+  constexpr int P = vpts_col(N);
+  auto kern = k_tz<R, N, P, STAGE>;
+  const size_t sh = t_lds<R, N>(kVCB);
+  static bool set = false;
+  if (!set) {
+    FF_TRY(t_attr(kern, sh));
+    set = true;
+  }
+  static const char* names[5] = {"tracer_z_stage0", "tracer_z_stage1", "tracer_z_stage2", "tracer_z_stage3",
+                                 "tracer_z_stage4"};
+  const double cs = sizeof(cplx<R>);
+  const double nk = (double)g.KX * g.KY;
+  const double bytes = (STAGE > 0 ? 3 : 0) * nk * g.nz * cs + (a.do_inverse ? nk * g.nz * cs : 0) +
+                       nk * g.KZ * (STAGE == 0 ? cs : (STAGE < 4 ? 3 * cs + 8 : 6 * cs + 8));
+  prof_begin(names[STAGE], bytes, st);
+  hipLaunchKernelGGL(kern, dim3(g.TX * g.KY), dim3(kVCB * (N / P)), sh, st, g, a);
+  prof_end(st);
+  FF_LAUNCHED("k_tz");
+  return FFHIP_OK;
+}
+
+template <typename R, int N, int DIR>
+static int launch_ty(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
+  constexpr int P = vpts_col(N);
+  auto kern = k_ty<R, N, P, DIR>;
+  const size_t sh = t_lds<R, N>(kVCB);
+  static bool set = false;
+  if (!set) {
+    FF_TRY(t_attr(kern, sh));
+    set = true;
+  }
+  const double cs = sizeof(cplx<R>);
+  const double nk = (double)g.KX * g.nz;
+  const double bytes = DIR == 1 ? nk * ((double)g.KY + g.ny) * cs : 3 * nk * ((double)g.ny + g.KY) * cs;
+  prof_begin(DIR == 1 ? "tracer_y_inverse" : "tracer_y_forward", bytes, st);
+  hipLaunchKernelGGL(kern, dim3(g.TX * g.nz), dim3(kVCB * (N / P)), sh, st, g, a);
+  prof_end(st);
+  FF_LAUNCHED("k_ty");
+  return FFHIP_OK;
+}
+
+template <typename R, int N>
+static int launch_tx(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
+  constexpr int P = vpts_row(N);
+  auto kern = k_tx<R, N, P>;
+  const size_t sh = (size_t)4 * TeamSched<N, P>::LDS * sizeof(cplx<R>);
+  static bool set = false;
+  if (!set) {
+    FF_TRY(t_attr(kern, sh));
+    set = true;
+  }
+  prof_begin("tracer_x", 4.0 * g.KX * (double)g.ny * g.nz * sizeof(cplx<R>), st);
+  hipLaunchKernelGGL(kern, dim3(g.nz * (g.ny / 4)), dim3(4 * (N / P)), sh, st, g, a);
+  prof_end(st);
+  FF_LAUNCHED("k_tx");
+  return FFHIP_OK;
+}
+
+template <typename R, int STAGE>
+static int disp_tz(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
+  switch (g.nz) {
+    case 64: return launch_tz<R, 64, STAGE>(g, a, st);
+    case 128: return launch_tz<R, 128, STAGE>(g, a, st);
+    case 256: return launch_tz<R, 256, STAGE>(g, a, st);
+    default: return launch_tz<R, 512, STAGE>(g, a, st);
+  }
+}
+
+template <typename R>
+static int disp_tz_stage(int stage, const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
+  switch (stage) {
+    case 0: return disp_tz<R, 0>(g, a, st);
+    case 1: return disp_tz<R, 1>(g, a, st);
+    case 2: return disp_tz<R, 2>(g, a, st);
+    case 3: return disp_tz<R, 3>(g, a, st);
+    default: return disp_tz<R, 4>(g, a, st);
+  }
+}
+
+template <typename R, int DIR>
+static int disp_ty(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
+  switch (g.ny) {
+    case 64: return launch_ty<R, 64, DIR>(g, a, st);
+    case 128: return launch_ty<R, 128, DIR>(g, a, st);
+    case 256: return launch_ty<R, 256, DIR>(g, a, st);
+    default: return launch_ty<R, 512, DIR>(g, a, st);
+  }
+}
+
+template <typename R>
+static int disp_tx(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
+  switch (g.nx) {
+    case 64: return launch_tx<R, 64>(g, a, st);
+    case 128: return launch_tx<R, 128>(g, a, st);
+    case 256: return launch_tx<R, 256>(g, a, st);
+    default: return launch_tx<R, 512>(g, a, st);
+  }
+}
+
+static void tstate_free(TracerState* s) {
+  if (!s) return;
+  void* ps[] = {s->S, s->S1, s->R[0], s->R[1], s->R[2], s->A1, s->A2, s->Bf[0], s->Bf[1], s->Bf[2],
+                s->Cf[0], s->Cf[1], s->Cf[2], s->Lp, s->twx, s->twy, s->twz};
+  for (void* p : ps) dev_free(p);
+  delete s;
+}
+
+int tracer_fused_create(ffhip_op* op) {
+  op->fused3 = nullptr;
+  if (op->kind != FFHIP_OP_TRACER3D) return FFHIP_OK;
+  const char* dis = getenv("FFHIP_DISABLE_FUSED");
+  if (dis && atoi(dis) != 0) return FFHIP_OK;
+  for (int d = 0; d < 3; ++d)
+    if (!tpow2(op->n[d]) || op->n[d] < 64 || op->n[d] > 512) return FFHIP_OK;
+  if (!(op->af > 0)) return FFHIP_OK;
+  TracerGeom g;
+  g.nx = (int)op->n[0];
+  g.ny = (int)op->n[1];
+  g.nz = (int)op->n[2];
+  g.nkr = (int)op->nkr;
+  g.KX = (int)op->lo[0] - 1;
+  g.KXp = (g.KX + kVCB - 1) / kVCB * kVCB;
+  g.TX = g.KXp / kVCB;
+  g.ylo = (int)op->lo[1] - 1;
+  g.yhi = (int)op->hi[1] - 1;
+  g.KY = g.ny - (g.yhi - g.ylo + 1);
+  g.zlo = (int)op->lo[2] - 1;
+  g.zhi = (int)op->hi[2] - 1;
+  g.KZ = g.nz - (g.zhi - g.zlo + 1);
+  const double twopi = 6.283185307179586476925286766559;
+  g.dk = twopi / op->box[0];
+  g.dl = twopi / op->box[1];
+  g.dm = twopi / op->box[2];
+  g.norm = 1.0 / ((double)g.nx * (double)g.ny * (double)g.nz);
+  g.A = op->params[0];
+  g.B = op->params[1];
+  g.C = op->params[2];
+  if (g.KX < 1 || g.KX > g.nx / 2 || g.ylo < 1 || g.yhi < g.ylo || g.ylo > g.ny / 2 || g.yhi < g.ny / 2 ||
+      g.zlo < 1 || g.zhi < g.zlo || g.zlo > g.nz / 2 || g.zhi < g.nz / 2)
+    return FFHIP_OK;
+  TracerState* s = new (std::nothrow) TracerState();
+  FF_CHECK(s, FFHIP_ERR_OOM, "host allocation failed");
+  s->g = g;
+  s->tab = reinterpret_cast<const double*>(op->tables);
+  const size_t cs = cplx_size(op->prec);
+  const size_t sb = (size_t)g.TX * g.KY * g.KZ * kVCB * cs;
+  const size_t ab = (size_t)g.nz * g.TX * g.ny * kVCB * cs;
+  const size_t cb = (size_t)g.TX * g.KY * g.nz * kVCB * cs;
+  int rc = dev_alloc(&s->S, sb);
+  if (rc == FFHIP_OK) rc = dev_alloc(&s->S1, sb);
+  for (int i = 0; i < 3 && rc == FFHIP_OK; ++i) rc = dev_alloc(&s->R[i], sb);
+  if (rc == FFHIP_OK) rc = dev_alloc(&s->A1, ab);  // zero-filled: its aliased y rows stay zero
+  if (rc == FFHIP_OK) rc = dev_alloc(&s->A2, ab);
+  for (int i = 0; i < 3 && rc == FFHIP_OK; ++i) rc = dev_alloc(&s->Bf[i], ab);
+  for (int i = 0; i < 3 && rc == FFHIP_OK; ++i) rc = dev_alloc(&s->Cf[i], cb);
+  if (rc == FFHIP_OK) rc = dev_alloc(&s->Lp, (size_t)g.TX * g.KY * g.KZ * kVCB * sizeof(double));
+  if (rc == FFHIP_OK) rc = upload_roots(g.nx, g.nx, op->prec, &s->twx);
+  if (rc == FFHIP_OK) rc = upload_roots(g.ny, g.ny, op->prec, &s->twy);
+  if (rc == FFHIP_OK) rc = upload_roots(g.nz, g.nz, op->prec, &s->twz);
+  if (rc != FFHIP_OK) {
+    tstate_free(s);
+    return rc;
+  }
+  op->fused3 = s;
+  return FFHIP_OK;
+}
+
+void tracer_fused_destroy(ffhip_op* op) {
+  tstate_free(reinterpret_cast<TracerState*>(op->fused3));
+  op->fused3 = nullptr;
+}
+
+template <typename R>
+static int trun(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t nsteps, hipStream_t st) {
+  TracerState* s = reinterpret_cast<TracerState*>(op->fused3);
+  const TracerGeom& g = s->g;
+  const int64_t n = (int64_t)g.TX * g.KY * g.KZ * kVCB;
+  if (s->lp_owner != ts) {
+    hipLaunchKernelGGL(k_tpackL, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<const double*>(ts->L), reinterpret_cast<double*>(s->Lp), g);
+    FF_LAUNCHED("k_tpackL");
+    s->lp_owner = ts;
+  }
+  hipLaunchKernelGGL((k_timport<R>), dim3(blocks_for(n, 256, 4096)), dim3(256), 0, st,
+                     reinterpret_cast<const cplx<R>*>(sol), reinterpret_cast<cplx<R>*>(s->S), g);
+  FF_LAUNCHED("k_timport");
+  TracerArgs a;
+  a.S = s->S;
+  a.S1 = s->S1;
+  a.R0 = s->R[0];
+  a.R1 = s->R[1];
+  a.R2 = s->R[2];
+  a.A1 = s->A1;
+  a.A2 = s->A2;
+  a.B0 = s->Bf[0];
+  a.B1 = s->Bf[1];
+  a.B2 = s->Bf[2];
+  a.C0 = s->Cf[0];
+  a.C1 = s->Cf[1];
+  a.C2 = s->Cf[2];
+  a.L = reinterpret_cast<const double*>(s->Lp);
+  a.tab = s->tab;
+  a.twx = s->twx;
+  a.twy = s->twy;
+  a.twz = s->twz;
+  // step sizes as rk4_run_generic takes them (dt already rounded to the state type)
+  a.h = dt;
+  a.h2 = op->prec == FFHIP_F32 ? (double)((float)dt / 2.0f) : dt / 2;
+  a.h6 = dt / 6;
+  a.do_inverse = 1;
+  FF_TRY(disp_tz_stage<R>(0, g, a, st));
+  for (int64_t k = 0; k < nsteps; ++k) {
+    for (int stage = 1; stage <= 4; ++stage) {
+      FF_TRY((disp_ty<R, 1>(g, a, st)));
+      FF_TRY(disp_tx<R>(g, a, st));
+      FF_TRY((disp_ty<R, -1>(g, a, st)));
+      a.do_inverse = (stage < 4 || k + 1 < nsteps) ? 1 : 0;
+      FF_TRY(disp_tz_stage<R>(stage, g, a, st));
+    }
+  }
+  const int64_t total = (int64_t)g.nz * g.ny * g.nkr;
+  hipLaunchKernelGGL((k_texport<R>), dim3(blocks_for(total, 256, 4096)), dim3(256), 0, st,
+                     reinterpret_cast<cplx<R>*>(sol), reinterpret_cast<const cplx<R>*>(s->S),
+                     reinterpret_cast<const double*>(ts->L), a.h2, a.h, a.h6, nsteps, g);
+  FF_LAUNCHED("k_texport");
+  return FFHIP_OK;
+}
+
+int tracer_fused_rk4_run(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t nsteps, hipStream_t st) {
+  FF_CHECK(op->fused3, FFHIP_ERR_UNSUPPORTED, "no fused tracer path for this operator");
+  return op->prec == FFHIP_F32 ? trun<float>(op, ts, sol, dt, nsteps, st) : trun<double>(op, ts, sol, dt, nsteps, st);
+}
+
+}  // namespace ffhip

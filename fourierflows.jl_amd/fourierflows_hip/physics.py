@@ -51,6 +51,13 @@ class BuiltinOperator:
     def __call__(self, N, sol, t, clock=None, vars=None, params=None, grid=None):
         _lib.call("ffhip_op_calcN", self._h, ptr(N), ptr(sol), float(t), stream_ptr())
 
+    @property
+    def fused(self):
+        """True when libffhip steps this operator's grid with fused kernels."""
+        v = C.c_int()
+        _lib.call("ffhip_op_is_fused", self._h, C.byref(v))
+        return bool(v.value)
+
     def diagnostics(self, sol):
         out = (C.c_double * 2)()
         _lib.call("ffhip_op_diagnostics", self._h, ptr(sol), out, stream_ptr())
@@ -74,13 +81,6 @@ class VorticityOperator(BuiltinOperator):
         if not isinstance(grid, TwoDGrid):
             raise TypeError("VorticityOperator needs a TwoDGrid")
         self._create(grid)
-
-    @property
-    def fused(self):
-        """True when libffhip runs this grid through the fused kernels."""
-        v = C.c_int()
-        _lib.call("ffhip_op_is_fused", self._h, C.byref(v))
-        return bool(v.value)
 
     def energy_enstrophy(self, sol):
         """(½⟨|u|²⟩, ½⟨ζ²⟩) by compensated FP64 Parseval sums."""

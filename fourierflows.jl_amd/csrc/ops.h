@@ -37,6 +37,7 @@ int fused_etdrk4_run(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, 
 int tracer_fused_create(ffhip_op* op);
 void tracer_fused_destroy(ffhip_op* op);
 int tracer_fused_rk4_run(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t nsteps, hipStream_t st);
+int tracer_fused_rk4_adjoint(ffhip_op* op, ffhip_rk4* ts, void* lam, double dt, int64_t nsteps, hipStream_t st);
 // discrete adjoint (vort2d_adjoint.hip)
 int fused_forward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void* solT, int64_t nsteps,
                        int64_t ncheck, hipStream_t st);

@@ -693,6 +693,8 @@ int ffhip_op_rk4_adjoint(ffhip_op* op, ffhip_rk4* ts, const void* lamT, void* la
   if (lam0 != lamT)
     FF_HIP(hipMemcpyAsync(lam0, lamT, (size_t)op->spec_count * cplx_size(op->prec), hipMemcpyDeviceToDevice, st));
   const double h = op->prec == FFHIP_F32 ? (double)(float)dt : dt;
+  if (nsteps == 0) return FFHIP_OK;
+  if (op->fused3 && !ts->filter) return tracer_fused_rk4_adjoint(op, ts, lam0, h, nsteps, st);
   return op->prec == FFHIP_F32 ? rk4_adjoint_tracer<float>(op, ts, lam0, h, nsteps, st)
                                : rk4_adjoint_tracer<double>(op, ts, lam0, h, nsteps, st);
 }

@@ -179,10 +179,19 @@ __global__ void __launch_bounds__(kVCB*(NZ / P)) k_tz(TracerGeom g, TracerArgs a
 }
 
 // ----------------------------------------------------------------- y passes
-// DIR = +1: inverse y-FFT of A1 (kept rows) into A2.  DIR = -1: forward y-FFTs
-// of the three products B into C (kept rows only).
-template <typename R, int NY, int P, int DIR>
-__global__ void __launch_bounds__(kVCB*(NY / P)) k_ty(TracerGeom g, TracerArgs a) {
+// NF fields, one y-column each: src rows read at the kept y only when
+// src_kept (the others are zero), dst written in full ([nz][TX][ny][4]) or
+// at the kept rows only into [TX][KY][nz][4] when dst_kept.  src == dst is
+// allowed (each column is read whole before it is written).
+struct TracerYArgs {
+  const void* src[3];
+  void* dst[3];
+  const void* tw;
+  int src_kept, dst_kept;
+};
+
+template <typename R, int NY, int P, int DIR, int NF>
+__global__ void __launch_bounds__(kVCB*(NY / P)) k_ty(TracerGeom g, TracerYArgs a) {
   constexpr int T = NY / P;
   constexpr int CB = kVCB;
   extern __shared__ unsigned char tsm[];
@@ -192,37 +201,28 @@ __global__ void __launch_bounds__(kVCB*(NY / P)) k_ty(TracerGeom g, TracerArgs a
   const int c = t % CB;
   const int tid = t / CB;
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * TeamSched<NY, P>::LDS;
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.twy);
+  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
   const int64_t base = ((int64_t)iz * g.TX + tx) * NY * CB + c;  // [z][tx][y][c]
-  if constexpr (DIR == 1) {
-    const cplx<R>* A1 = reinterpret_cast<const cplx<R>*>(a.A1);
-    cplx<R>* A2 = reinterpret_cast<cplx<R>*>(a.A2);
+#pragma unroll 1
+  for (int f = 0; f < NF; ++f) {
+    const cplx<R>* src = reinterpret_cast<const cplx<R>*>(a.src[f]);
+    cplx<R>* dst = reinterpret_cast<cplx<R>*>(a.dst[f]);
     cplx<R> v[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       const int iy = tid + p * T;
-      v[p] = (iy < g.ylo || iy > g.yhi) ? A1[base + (int64_t)iy * CB] : mkc<R>(0, 0);
+      const bool zero = a.src_kept && iy >= g.ylo && iy <= g.yhi;
+      v[p] = zero ? mkc<R>(0, 0) : src[base + (int64_t)iy * CB];
     }
-    team_fft<NY, P, 1, R>(v, lds, tid, tw);
+    team_fft<NY, P, DIR, R>(v, lds, tid, tw);
 #pragma unroll
-    for (int p = 0; p < P; ++p) A2[base + (int64_t)(tid + p * T) * CB] = v[p];
-  } else {
-    const cplx<R>* Bs[3] = {reinterpret_cast<const cplx<R>*>(a.B0), reinterpret_cast<const cplx<R>*>(a.B1),
-                            reinterpret_cast<const cplx<R>*>(a.B2)};
-    cplx<R>* Cs[3] = {reinterpret_cast<cplx<R>*>(a.C0), reinterpret_cast<cplx<R>*>(a.C1),
-                      reinterpret_cast<cplx<R>*>(a.C2)};
-#pragma unroll 1
-    for (int f = 0; f < 3; ++f) {
-      cplx<R> v[P];
-#pragma unroll
-      for (int p = 0; p < P; ++p) v[p] = Bs[f][base + (int64_t)(tid + p * T) * CB];
-      team_fft<NY, P, -1, R>(v, lds, tid, tw);
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-        const int iy = tid + p * T;
-        if (iy >= g.ylo && iy <= g.yhi) continue;
+    for (int p = 0; p < P; ++p) {
+      const int iy = tid + p * T;
+      if (!a.dst_kept) {
+        dst[base + (int64_t)iy * CB] = v[p];
+      } else if (iy < g.ylo || iy > g.yhi) {
         const int ky = t_kept_index(iy, g.ylo, g.yhi);
-        Cs[f][(((int64_t)tx * g.KY + ky) * g.nz + iz) * CB + c] = v[p];
+        dst[(((int64_t)tx * g.KY + ky) * g.nz + iz) * CB + c] = v[p];
       }
     }
   }
@@ -313,6 +313,231 @@ __global__ void __launch_bounds__(4 * (NX / P)) k_tx(TracerGeom g, TracerArgs a)
     const int k = tid + p * T;
     if (k >= g.KXp) continue;
     B2[xaddr(k)] = k < KX ? w[p] : mkc<R>(0, 0);
+  }
+}
+
+// ----------------------------------------------------------------- adjoint passes
+// Transpose of the RK4 step (the tracer is linear in c, so no forward states
+// are needed), with the arithmetic of the library's generic tracer adjoint:
+//   (N + L)^T x = P (W/Nt) mul!( u p + v q + w r ) + L x,
+//   p, q, r = unnormalised c2r of i kx x / W, i l x / W, i m x / W (x masked),
+// W = 1 on the kx = 0 column, 2 elsewhere.  Passes per evaluation:
+//   z pass   : forward z-FFT of the previous evaluation's product spectrum ->
+//              its output o -> the reverse RK4 combination -> the next input
+//              kb -> inverse z-FFTs of its three fields
+//   y-inverse: the three fields, in place
+//   x pass   : c2r of p + i q (packed) and of r, the product u p + v q + w r,
+//              its forward x-FFT
+//   y-forward: the product, kept rows only
+// Reverse step (lam the adjoint state, h6 = dt/6, h3 = 2 dt/6, h2 = dt/2):
+//   kb = h6 lam;  o = f(kb); ub = lam + o;  kb = h3 lam + h o
+//   o = f(kb); ub += o; kb = h3 lam + h2 o; o = f(kb); ub += o; kb = h6 lam + h2 o
+//   o = f(kb); lam = ub + o
+struct TracerAdjArgs {
+  void* LAM;  // [TX][KY][KZ][4]
+  void* UB;
+  void* KB;
+  const void* F;  // forward z input of the product spectrum, [TX][KY][nz][4]
+  void* Z0;       // inverse z outputs of the three fields, [nz][TX][ny][4]
+  void* Z1;
+  void* Z2;
+  const double* L;
+  const void* twz;
+  double h, h2, h3, h6;
+  int cont;  // STAGE 4: 1 = start the next (earlier) step, 0 = the run ends here
+};
+
+template <typename R, int NZ, int P, int STAGE>
+__global__ void __launch_bounds__(kVCB*(NZ / P)) k_taz(TracerGeom g, TracerAdjArgs a) {
+  constexpr int T = NZ / P;
+  constexpr int CB = kVCB;
+  extern __shared__ unsigned char tsm[];
+  const int grp = (int)blockIdx.x;
+  const int tx = grp / g.KY, ky = grp % g.KY;
+  const int t = threadIdx.x;
+  const int c = t % CB;
+  const int tid = t / CB;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * TeamSched<NZ, P>::LDS;
+  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.twz);
+  const int ix = tx * CB + c;
+  const bool colkept = ix < g.KX;
+  const int iy = t_full_index(ky, g.ylo, g.yhi);
+  const int64_t fbase = ((int64_t)tx * g.KY + ky) * NZ * CB + c;
+  const int64_t sbase = ((int64_t)tx * g.KY + ky) * g.KZ * CB + c;
+
+  cplx<R> fz[P];
+  if constexpr (STAGE > 0) {
+    const cplx<R>* F = reinterpret_cast<const cplx<R>*>(a.F);
+#pragma unroll
+    for (int p = 0; p < P; ++p) fz[p] = F[fbase + (int64_t)(tid + p * T) * CB];
+    team_fft<NZ, P, -1, R>(fz, lds, tid, tw);
+  }
+  cplx<R>* LAM = reinterpret_cast<cplx<R>*>(a.LAM);
+  cplx<R>* UB = reinterpret_cast<cplx<R>*>(a.UB);
+  cplx<R>* KB = reinterpret_cast<cplx<R>*>(a.KB);
+  const double wn = (ix == 0 ? 1.0 : 2.0) * g.norm;  // W / Nt
+  cplx<R> kb[P];
+  bool next = true;
+  if constexpr (STAGE == 4) next = a.cont != 0;
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int iz = tid + p * T;
+    kb[p] = mkc<R>(0, 0);
+    const bool kept = colkept && (iz < g.zlo || iz > g.zhi);
+    if (!kept) continue;
+    const int64_t si = sbase + (int64_t)t_kept_index(iz, g.zlo, g.zhi) * CB;
+    const cplx<R> lam = LAM[si];
+    if constexpr (STAGE == 0) {
+      kb[p] = mkc<R>((R)(a.h6 * (double)lam.x), (R)(a.h6 * (double)lam.y));
+    } else {
+      const cplx<R> x = KB[si];
+      const double l = a.L[si];
+      const cplx<R> o = mkc<R>((R)(l * (double)x.x + wn * (double)fz[p].x), (R)(l * (double)x.y + wn * (double)fz[p].y));
+      if constexpr (STAGE < 4) {
+        const cplx<R> base = STAGE == 1 ? lam : UB[si];
+        UB[si] = mkc<R>(base.x + o.x, base.y + o.y);
+        const double cl = STAGE == 3 ? a.h6 : a.h3, cu = STAGE == 1 ? a.h : a.h2;
+        kb[p] = mkc<R>((R)(cl * (double)lam.x + cu * (double)o.x), (R)(cl * (double)lam.y + cu * (double)o.y));
+      } else {
+        const cplx<R> ub = UB[si];
+        const cplx<R> nl = mkc<R>(ub.x + o.x, ub.y + o.y);
+        LAM[si] = nl;
+        if (next) kb[p] = mkc<R>((R)(a.h6 * (double)nl.x), (R)(a.h6 * (double)nl.y));
+      }
+    }
+    if (next) KB[si] = kb[p];
+  }
+  if (!next) return;  // uniform over the block
+
+  // inverse z-FFTs of i kx kb / W, i l kb / W, i m kb / W
+  const double iw = ix == 0 ? 1.0 : 0.5;
+  const R qx = (R)(g.dk * (double)ix * iw), qy = (R)(g.dl * (double)vfreq(iy, g.ny) * iw);
+  void* dsts[3] = {a.Z0, a.Z1, a.Z2};
+  const int64_t plane = (int64_t)g.TX * g.ny * CB;
+  const int64_t abase = ((int64_t)tx * g.ny + iy) * CB + c;
+#pragma unroll 1
+  for (int f = 0; f < 3; ++f) {
+    cplx<R> v[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int iz = tid + p * T;
+      const R q = f == 0 ? qx : (f == 1 ? qy : (R)(g.dm * (double)vfreq(iz, NZ) * iw));
+      v[p] = mkc<R>(-q * kb[p].y, q * kb[p].x);
+    }
+    team_fft<NZ, P, 1, R>(v, lds, tid, tw);
+    cplx<R>* d = reinterpret_cast<cplx<R>*>(dsts[f]);
+#pragma unroll
+    for (int p = 0; p < P; ++p) d[(int64_t)(tid + p * T) * plane + abase] = v[p];
+  }
+}
+
+// adjoint x pass: src fields (p, q, r half spectra after the y-inverse) -> product spectrum in dst
+template <typename R, int NX, int P>
+__global__ void __launch_bounds__(4 * (NX / P)) k_tax(TracerGeom g, const void* s0, const void* s1, const void* s2,
+                                                      void* dst, const double* tab, const void* twx) {
+  constexpr int T = NX / P;
+  constexpr int CB = kVCB;
+  extern __shared__ unsigned char tsm[];
+  const int grp = (int)blockIdx.x;
+  const int iz = grp / (g.ny / 4);
+  const int r = (int)(threadIdx.x % 4);
+  const int y = (grp % (g.ny / 4)) * 4 + r;
+  const int tid = threadIdx.x / 4;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + r * TeamSched<NX, P>::LDS;
+  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(twx);
+  const int KX = g.KX;
+  const int64_t ny = g.ny;
+  const int64_t zoff = (int64_t)iz * g.TX * ny * CB;
+  auto xaddr = [&](int k) -> int64_t { return zoff + ((int64_t)(k / CB) * ny + y) * CB + (k % CB); };
+  const cplx<R>* Ph = reinterpret_cast<const cplx<R>*>(s0);
+  const cplx<R>* Qh = reinterpret_cast<const cplx<R>*>(s1);
+  const cplx<R>* Rh = reinterpret_cast<const cplx<R>*>(s2);
+  cplx<R> z[P], w[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int k = tid + p * T;
+    cplx<R> sv = mkc<R>(0, 0), sr = mkc<R>(0, 0);
+    if (k == 0) {
+      sv = mkc<R>(Ph[xaddr(0)].x, Qh[xaddr(0)].x);  // c2r ignores Im at kx = 0
+      sr = mkc<R>(Rh[xaddr(0)].x, 0);
+    } else if (k < KX) {
+      const cplx<R> A = Ph[xaddr(k)], B = Qh[xaddr(k)];
+      sv = mkc<R>(A.x - B.y, A.y + B.x);
+      sr = Rh[xaddr(k)];
+    } else if (k > NX - KX) {
+      const cplx<R> A = Ph[xaddr(NX - k)], B = Qh[xaddr(NX - k)];
+      sv = mkc<R>(A.x + B.y, B.x - A.y);
+      sr = cconj<R>(Rh[xaddr(NX - k)]);
+    }
+    z[p] = sv;
+    w[p] = sr;
+  }
+  team_fft<NX, P, 1, R>(z, lds, tid, tw);  // p + i q
+  team_fft<NX, P, 1, R>(w, lds, tid, tw);  // r
+  const double* sx = tab;
+  const double* cx = tab + g.nx;
+  const double* sy = tab + 2 * g.nx;
+  const double* cy = sy + g.ny;
+  const double* sz = tab + 2 * g.nx + 2 * g.ny;
+  const double* cz = sz + g.nz;
+  const R u = (R)(g.A * sz[iz] + g.C * cy[y]);
+  const double acz = g.A * cz[iz], csy = g.C * sy[y];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int ix = tid + p * T;
+    const R v = (R)(g.B * sx[ix] + acz);
+    const R ww = (R)(csy + g.B * cx[ix]);
+    z[p] = mkc<R>(u * z[p].x + v * z[p].y + ww * w[p].x, 0);  // u p + v q + w r
+  }
+  team_fft<NX, P, -1, R>(z, lds, tid, tw);
+  cplx<R>* D = reinterpret_cast<cplx<R>*>(dst);
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int k = tid + p * T;
+    if (k >= g.KXp) continue;
+    D[xaddr(k)] = k < KX ? z[p] : mkc<R>(0, 0);
+  }
+}
+
+// aliased modes of the adjoint state: the reverse RK4 combination with f(x) = L x
+template <typename R>
+__global__ void k_taexport(cplx<R>* __restrict__ lam, const cplx<R>* __restrict__ S, const double* __restrict__ L,
+                           double h2, double h, double h3, double h6, int64_t nsteps, TracerGeom g) {
+  const int64_t total = (int64_t)g.nz * g.ny * g.nkr;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(idx % g.nkr);
+    const int64_t r = idx / g.nkr;
+    const int iy = (int)(r % g.ny), iz = (int)(r / g.ny);
+    const bool kept = ix < g.KX && (iy < g.ylo || iy > g.yhi) && (iz < g.zlo || iz > g.zhi);
+    if (kept) {
+      const int tx = ix / kVCB, c = ix % kVCB;
+      const int ky = t_kept_index(iy, g.ylo, g.yhi), kz = t_kept_index(iz, g.zlo, g.zhi);
+      lam[idx] = S[(((int64_t)tx * g.KY + ky) * g.KZ + kz) * kVCB + c];
+      continue;
+    }
+    cplx<R> u = lam[idx];
+    if (u.x == R(0) && u.y == R(0)) continue;
+    const double l = L[idx];
+    auto f = [&](cplx<R> x) { return mkc<R>((R)(l * (double)x.x), (R)(l * (double)x.y)); };
+    auto lin = [&](cplx<R> a, double ca, cplx<R> b, double cb) {
+      return mkc<R>((R)(ca * (double)a.x + cb * (double)b.x), (R)(ca * (double)a.y + cb * (double)b.y));
+    };
+    for (int64_t s = 0; s < nsteps; ++s) {
+      cplx<R> kb = mkc<R>((R)(h6 * (double)u.x), (R)(h6 * (double)u.y));
+      cplx<R> o = f(kb);
+      cplx<R> ub = mkc<R>(u.x + o.x, u.y + o.y);
+      kb = lin(u, h3, o, h);
+      o = f(kb);
+      ub = mkc<R>(ub.x + o.x, ub.y + o.y);
+      kb = lin(u, h3, o, h2);
+      o = f(kb);
+      ub = mkc<R>(ub.x + o.x, ub.y + o.y);
+      kb = lin(u, h6, o, h2);
+      o = f(kb);
+      u = mkc<R>(ub.x + o.x, ub.y + o.y);
+    }
+    lam[idx] = u;
   }
 }
 
@@ -429,10 +654,10 @@ static int launch_tz(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
   return FFHIP_OK;
 }
 
-template <typename R, int N, int DIR>
-static int launch_ty(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
+template <typename R, int N, int DIR, int NF>
+static int launch_ty(const TracerGeom& g, const TracerYArgs& a, const char* name, hipStream_t st) {
   constexpr int P = vpts_col(N);
-  auto kern = k_ty<R, N, P, DIR>;
+  auto kern = k_ty<R, N, P, DIR, NF>;
   const size_t sh = t_lds<R, N>(kVCB);
   static bool set = false;
   if (!set) {
@@ -441,8 +666,8 @@ static int launch_ty(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
   }
   const double cs = sizeof(cplx<R>);
   const double nk = (double)g.KX * g.nz;
-  const double bytes = DIR == 1 ? nk * ((double)g.KY + g.ny) * cs : 3 * nk * ((double)g.ny + g.KY) * cs;
-  prof_begin(DIR == 1 ? "tracer_y_inverse" : "tracer_y_forward", bytes, st);
+  const double bytes = NF * nk * ((a.src_kept ? g.KY : g.ny) + (a.dst_kept ? g.KY : g.ny)) * cs;
+  prof_begin(name, bytes, st);
   hipLaunchKernelGGL(kern, dim3(g.TX * g.nz), dim3(kVCB * (N / P)), sh, st, g, a);
   prof_end(st);
   FF_LAUNCHED("k_ty");
@@ -487,14 +712,29 @@ static int disp_tz_stage(int stage, const TracerGeom& g, const TracerArgs& a, hi
   }
 }
 
-template <typename R, int DIR>
-static int disp_ty(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
+template <typename R, int DIR, int NF>
+static int disp_ty(const TracerGeom& g, const TracerYArgs& a, const char* name, hipStream_t st) {
   switch (g.ny) {
-    case 64: return launch_ty<R, 64, DIR>(g, a, st);
-    case 128: return launch_ty<R, 128, DIR>(g, a, st);
-    case 256: return launch_ty<R, 256, DIR>(g, a, st);
-    default: return launch_ty<R, 512, DIR>(g, a, st);
+    case 64: return launch_ty<R, 64, DIR, NF>(g, a, name, st);
+    case 128: return launch_ty<R, 128, DIR, NF>(g, a, name, st);
+    case 256: return launch_ty<R, 256, DIR, NF>(g, a, name, st);
+    default: return launch_ty<R, 512, DIR, NF>(g, a, name, st);
   }
+}
+
+static TracerYArgs yargs(const void* s0, const void* s1, const void* s2, void* d0, void* d1, void* d2,
+                         const void* tw, int src_kept, int dst_kept) {
+  TracerYArgs y;
+  y.src[0] = s0;
+  y.src[1] = s1;
+  y.src[2] = s2;
+  y.dst[0] = d0;
+  y.dst[1] = d1;
+  y.dst[2] = d2;
+  y.tw = tw;
+  y.src_kept = src_kept;
+  y.dst_kept = dst_kept;
+  return y;
 }
 
 template <typename R>
@@ -504,6 +744,77 @@ static int disp_tx(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
     case 128: return launch_tx<R, 128>(g, a, st);
     case 256: return launch_tx<R, 256>(g, a, st);
     default: return launch_tx<R, 512>(g, a, st);
+  }
+}
+
+template <typename R, int N, int STAGE>
+static int launch_taz(const TracerGeom& g, const TracerAdjArgs& a, hipStream_t st) {
+  constexpr int P = vpts_col(N);
+  auto kern = k_taz<R, N, P, STAGE>;
+  const size_t sh = t_lds<R, N>(kVCB);
+  static bool set = false;
+  if (!set) {
+    FF_TRY(t_attr(kern, sh));
+    set = true;
+  }
+  static const char* names[5] = {"tracer_adj_z_stage0", "tracer_adj_z_stage1", "tracer_adj_z_stage2",
+                                 "tracer_adj_z_stage3", "tracer_adj_z_stage4"};
+  const double cs = sizeof(cplx<R>);
+  const double nk = (double)g.KX * g.KY;
+  const double bytes = (STAGE > 0 ? nk * g.nz * cs : 0) + 3 * nk * g.nz * cs + nk * g.KZ * (4 * cs + 8);
+  prof_begin(names[STAGE], bytes, st);
+  hipLaunchKernelGGL(kern, dim3(g.TX * g.KY), dim3(kVCB * (N / P)), sh, st, g, a);
+  prof_end(st);
+  FF_LAUNCHED("k_taz");
+  return FFHIP_OK;
+}
+
+template <typename R, int STAGE>
+static int disp_taz(const TracerGeom& g, const TracerAdjArgs& a, hipStream_t st) {
+  switch (g.nz) {
+    case 64: return launch_taz<R, 64, STAGE>(g, a, st);
+    case 128: return launch_taz<R, 128, STAGE>(g, a, st);
+    case 256: return launch_taz<R, 256, STAGE>(g, a, st);
+    default: return launch_taz<R, 512, STAGE>(g, a, st);
+  }
+}
+
+template <typename R>
+static int disp_taz_stage(int stage, const TracerGeom& g, const TracerAdjArgs& a, hipStream_t st) {
+  switch (stage) {
+    case 0: return disp_taz<R, 0>(g, a, st);
+    case 1: return disp_taz<R, 1>(g, a, st);
+    case 2: return disp_taz<R, 2>(g, a, st);
+    case 3: return disp_taz<R, 3>(g, a, st);
+    default: return disp_taz<R, 4>(g, a, st);
+  }
+}
+
+template <typename R, int N>
+static int launch_tax(const TracerGeom& g, TracerState* s, hipStream_t st) {
+  constexpr int P = vpts_row(N);
+  auto kern = k_tax<R, N, P>;
+  const size_t sh = (size_t)4 * TeamSched<N, P>::LDS * sizeof(cplx<R>);
+  static bool set = false;
+  if (!set) {
+    FF_TRY(t_attr(kern, sh));
+    set = true;
+  }
+  prof_begin("tracer_adj_x", 4.0 * g.KX * (double)g.ny * g.nz * sizeof(cplx<R>), st);
+  hipLaunchKernelGGL(kern, dim3(g.nz * (g.ny / 4)), dim3(4 * (N / P)), sh, st, g, (const void*)s->Bf[0],
+                     (const void*)s->Bf[1], (const void*)s->Bf[2], s->A2, s->tab, (const void*)s->twx);
+  prof_end(st);
+  FF_LAUNCHED("k_tax");
+  return FFHIP_OK;
+}
+
+template <typename R>
+static int disp_tax(const TracerGeom& g, TracerState* s, hipStream_t st) {
+  switch (g.nx) {
+    case 64: return launch_tax<R, 64>(g, s, st);
+    case 128: return launch_tax<R, 128>(g, s, st);
+    case 256: return launch_tax<R, 256>(g, s, st);
+    default: return launch_tax<R, 512>(g, s, st);
   }
 }
 
@@ -618,12 +929,14 @@ static int trun(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t nstep
   a.h2 = op->prec == FFHIP_F32 ? (double)((float)dt / 2.0f) : dt / 2;
   a.h6 = dt / 6;
   a.do_inverse = 1;
+  const TracerYArgs yinv = yargs(s->A1, nullptr, nullptr, s->A2, nullptr, nullptr, s->twy, 1, 0);
+  const TracerYArgs yfwd = yargs(s->Bf[0], s->Bf[1], s->Bf[2], s->Cf[0], s->Cf[1], s->Cf[2], s->twy, 0, 1);
   FF_TRY(disp_tz_stage<R>(0, g, a, st));
   for (int64_t k = 0; k < nsteps; ++k) {
     for (int stage = 1; stage <= 4; ++stage) {
-      FF_TRY((disp_ty<R, 1>(g, a, st)));
+      FF_TRY((disp_ty<R, 1, 1>(g, yinv, "tracer_y_inverse", st)));
       FF_TRY(disp_tx<R>(g, a, st));
-      FF_TRY((disp_ty<R, -1>(g, a, st)));
+      FF_TRY((disp_ty<R, -1, 3>(g, yfwd, "tracer_y_forward", st)));
       a.do_inverse = (stage < 4 || k + 1 < nsteps) ? 1 : 0;
       FF_TRY(disp_tz_stage<R>(stage, g, a, st));
     }
@@ -634,6 +947,62 @@ static int trun(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t nstep
                      reinterpret_cast<const double*>(ts->L), a.h2, a.h, a.h6, nsteps, g);
   FF_LAUNCHED("k_texport");
   return FFHIP_OK;
+}
+
+// lam <- (d sol_T / d sol_0)^T lam over nsteps fused RK4 steps
+template <typename R>
+static int tadj(ffhip_op* op, ffhip_rk4* ts, void* lam, double dt, int64_t nsteps, hipStream_t st) {
+  TracerState* s = reinterpret_cast<TracerState*>(op->fused3);
+  const TracerGeom& g = s->g;
+  const int64_t n = (int64_t)g.TX * g.KY * g.KZ * kVCB;
+  if (s->lp_owner != ts) {
+    hipLaunchKernelGGL(k_tpackL, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<const double*>(ts->L), reinterpret_cast<double*>(s->Lp), g);
+    FF_LAUNCHED("k_tpackL");
+    s->lp_owner = ts;
+  }
+  hipLaunchKernelGGL((k_timport<R>), dim3(blocks_for(n, 256, 4096)), dim3(256), 0, st,
+                     reinterpret_cast<const cplx<R>*>(lam), reinterpret_cast<cplx<R>*>(s->S), g);
+  FF_LAUNCHED("k_timport");
+  TracerAdjArgs a;
+  a.LAM = s->S;
+  a.UB = s->S1;
+  a.KB = s->R[0];
+  a.F = s->Cf[0];
+  a.Z0 = s->Bf[0];
+  a.Z1 = s->Bf[1];
+  a.Z2 = s->Bf[2];
+  a.L = reinterpret_cast<const double*>(s->Lp);
+  a.twz = s->twz;
+  a.h = dt;
+  a.h2 = op->prec == FFHIP_F32 ? (double)((float)dt / 2.0f) : dt / 2;
+  a.h6 = dt / 6;
+  a.h3 = 2 * (dt / 6);
+  a.cont = 1;
+  // y-inverse of the three fields in place; y-forward of the product into F
+  const TracerYArgs yinv = yargs(s->Bf[0], s->Bf[1], s->Bf[2], s->Bf[0], s->Bf[1], s->Bf[2], s->twy, 1, 0);
+  const TracerYArgs yfwd = yargs(s->A2, nullptr, nullptr, s->Cf[0], nullptr, nullptr, s->twy, 0, 1);
+  FF_TRY(disp_taz_stage<R>(0, g, a, st));
+  for (int64_t k = 0; k < nsteps; ++k) {
+    for (int stage = 1; stage <= 4; ++stage) {
+      FF_TRY((disp_ty<R, 1, 3>(g, yinv, "tracer_adj_y_inverse", st)));
+      FF_TRY(disp_tax<R>(g, s, st));
+      FF_TRY((disp_ty<R, -1, 1>(g, yfwd, "tracer_adj_y_forward", st)));
+      a.cont = (stage < 4 || k + 1 < nsteps) ? 1 : 0;
+      FF_TRY(disp_taz_stage<R>(stage, g, a, st));
+    }
+  }
+  const int64_t total = (int64_t)g.nz * g.ny * g.nkr;
+  hipLaunchKernelGGL((k_taexport<R>), dim3(blocks_for(total, 256, 4096)), dim3(256), 0, st,
+                     reinterpret_cast<cplx<R>*>(lam), reinterpret_cast<const cplx<R>*>(s->S),
+                     reinterpret_cast<const double*>(ts->L), a.h2, a.h, a.h3, a.h6, nsteps, g);
+  FF_LAUNCHED("k_taexport");
+  return FFHIP_OK;
+}
+
+int tracer_fused_rk4_adjoint(ffhip_op* op, ffhip_rk4* ts, void* lam, double dt, int64_t nsteps, hipStream_t st) {
+  FF_CHECK(op->fused3, FFHIP_ERR_UNSUPPORTED, "no fused tracer path for this operator");
+  return op->prec == FFHIP_F32 ? tadj<float>(op, ts, lam, dt, nsteps, st) : tadj<double>(op, ts, lam, dt, nsteps, st);
 }
 
 int tracer_fused_rk4_run(ffhip_op* op, ffhip_rk4* ts, void* sol, double dt, int64_t nsteps, hipStream_t st) {

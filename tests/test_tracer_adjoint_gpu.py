@@ -138,3 +138,37 @@ def test_fp32_gradient_256_300_steps(ff):
     print(f"tracer 256^3: J fp32 {J32!r} fp64 {J64!r} gradient rel l2 {err:.3e}")
     assert abs(J32 - J64) < 1e-4 * J64, (J32, J64)
     assert err < 1e-4, err
+
+
+def test_fused_adjoint_dot_product_and_generic(ff, monkeypatch):
+    """The fused RK4 tracer adjoint (64^3, power of two) satisfies the
+    dot-product identity with the fused forward map and equals the generic
+    adjoint of the library (FP64)."""
+    import torch
+
+    from fourierflows_hip import _lib
+
+    n, nsteps = 64, 5
+    pf = ff.TracerProblem(n=n, kappa=1e-2, dt=0.01, T=np.float64)
+    monkeypatch.setenv("FFHIP_DISABLE_FUSED", "1")
+    pg = ff.TracerProblem(n=n, kappa=1e-2, dt=0.01, T=np.float64)
+    monkeypatch.delenv("FFHIP_DISABLE_FUSED", raising=False)
+    assert pf.eqn.calcN.fused and not pg.eqn.calcN.fused
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    d = torch.randn(pf.sol.shape, generator=gen, device="cuda", dtype=torch.complex128)
+    lam = torch.randn(pf.sol.shape, generator=gen, device="cuda", dtype=torch.complex128)
+    ff.dealias(d, pf.grid)
+    st = torch.cuda.current_stream().cuda_stream
+    Sd = d.clone()
+    _lib.call("ffhip_op_rk4_run", pf.eqn.calcN.handle, pf.timestepper.handle, Sd.data_ptr(), pf.clock.dt, nsteps, st)
+    out = {}
+    for name, p in (("fused", pf), ("generic", pg)):
+        o = torch.zeros_like(lam)
+        _lib.call("ffhip_op_rk4_adjoint", p.eqn.calcN.handle, p.timestepper.handle, lam.data_ptr(), o.data_ptr(),
+                  p.clock.dt, nsteps, st)
+        out[name] = o
+    lhs = float(torch.sum(Sd.real * lam.real + Sd.imag * lam.imag))
+    rhs = float(torch.sum(d.real * out["fused"].real + d.imag * out["fused"].imag))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs), (lhs, rhs)
+    err = float(torch.linalg.vector_norm(out["fused"] - out["generic"]) / torch.linalg.vector_norm(out["generic"]))
+    assert err < 1e-12, err

@@ -11,6 +11,8 @@ include/ffhip.h) that keeps the reference's API names:
   (``makefilter``), ``stepforward`` (``stepforward!``), ``step_until``
                                            (src/problem.jl, src/timesteppers.jl)
 * ``parsevalsum2``, ``parsevalsum``               (src/utils.jl)
+* ``Diagnostic``, ``increment``, ``update`` and ``stepforward(prob, diags,
+  nsteps)``                                        (src/diagnostics.jl)
 * built-in operators ``VorticityOperator`` / ``TracerOperator`` and the
   ``Diffusion`` testbed                              (src/diffusion.jl)
 * ``adjoint``: differentiable ``ldiv``/``mul``/``dealias`` and the discrete
@@ -20,7 +22,8 @@ Device arrays are torch tensors on the ROCm device (``HIP()`` is the device
 tag, the analogue of ``GPU()``).
 """
 
-from . import _lib, adjoint
+from . import _lib, adjoint, diagnostics
+from .diagnostics import Diagnostic, increment, update
 from ._lib import DomainError, FFHIPError, NonFiniteError
 from .core import CPU, GPU, HIP, cxtype, device_array, fltype, init, zeros
 from .domains import (OneDGrid, Plan, ThreeDGrid, TwoDGrid, dealias, getaliasedwavenumbers, gridpoints, ldiv,

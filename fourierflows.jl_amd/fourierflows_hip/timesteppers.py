@@ -405,13 +405,42 @@ class Problem:
         self.sol = zeros(dev, eqn.T, eqn.dims)
 
 
-def stepforward(prob, nsteps=1):
-    """``stepforward!(prob, nsteps)`` (src/timesteppers.jl:6-20).
+def stepforward(prob, *args):
+    """``stepforward!(prob[, diags], nsteps)`` (src/timesteppers.jl:6-35).
 
-    With a built-in operator as ``calcN!`` all ``nsteps`` run inside
-    libffhip (fused kernels, no host round trips); otherwise each step calls
-    the Python ``calcN!`` between library pointwise passes.
+    With a built-in operator as ``calcN!`` the steps run inside libffhip
+    (fused kernels, no host round trips); otherwise each step calls the
+    Python ``calcN!`` between library pointwise passes.  With ``diags`` (a
+    ``Diagnostic`` or a list of them) the diagnostics are incremented after
+    every step as the reference does; with a built-in operator the library
+    runs the steps between two diagnostic updates as one call.
     """
+    if len(args) == 0:
+        diags, nsteps = None, 1
+    elif len(args) == 1:
+        diags, nsteps = None, args[0]
+    else:
+        diags, nsteps = args
+    nsteps = int(nsteps)
+    if diags is not None:
+        from .diagnostics import _as_list, increment
+
+        dl = _as_list(diags)
+        op = getattr(prob.eqn.calcN, "builtin_operator", None)
+        if op is None:
+            for _ in range(nsteps):
+                stepforward(prob, 1)
+                increment(dl)
+            return None
+        done = 0
+        while done < nsteps:
+            # steps until the next update of any diagnostic (or the end)
+            step = prob.clock.step
+            chunk = min(min(d.freq - (step % d.freq) for d in dl), nsteps - done)
+            stepforward(prob, chunk)
+            done += chunk
+            increment(dl)
+        return None
     op = getattr(prob.eqn.calcN, "builtin_operator", None)
     if op is not None and nsteps > 0:
         prob.timestepper.run_fused(op, prob.sol, prob.clock, nsteps)

@@ -161,3 +161,41 @@ def test_oracle_not_imported_by_product():
         if f.endswith(".py"):
             src = open(os.path.join(pkg, f)).read()
             assert not re.search(r"^\s*(import|from)\s+oracle\b", src, flags=re.M), f
+
+
+def test_diagnostic_chunking_host():
+    """stepforward(prob, diags, n) with a built-in operator runs the steps
+    between diagnostic updates as single library calls, and the saved steps
+    are those of the reference's step-by-step loop (src/timesteppers.jl:28-35,
+    src/diagnostics.jl:86-104)."""
+    import types
+
+    from fourierflows_hip import diagnostics as fd
+    from fourierflows_hip import timesteppers as fts
+
+    calls = []
+
+    class FakeTS:
+        def run_fused(self, op, sol, clock, n):
+            calls.append(n)
+            clock.step += n
+            clock.t += n * 0.5
+
+    clock = types.SimpleNamespace(step=0, t=0.0)
+    calcN = types.SimpleNamespace(builtin_operator=object())
+    prob = types.SimpleNamespace(clock=clock, timestepper=FakeTS(), sol=None,
+                                 eqn=types.SimpleNamespace(calcN=calcN))
+    d3 = fd.Diagnostic(lambda p: p.clock.step, prob, nsteps=20, freq=3)
+    d4 = fd.Diagnostic(lambda p: p.clock.t, prob, nsteps=20, freq=4)
+    fts.stepforward(prob, [d3, d4], 20)
+    assert sum(calls) == 20 and calls == [3, 1, 2, 2, 1, 3, 3, 1, 2, 2]
+    assert list(d3["steps"]) == [0, 3, 6, 9, 12, 15, 18]
+    assert list(d3["data"]) == [0, 3, 6, 9, 12, 15, 18]
+    assert list(d4["steps"]) == [0, 4, 8, 12, 16, 20]
+    assert list(d4["t"]) == [0.0, 2.0, 4.0, 6.0, 8.0, 10.0]
+    # a second call continues from the clock's step
+    calls.clear()
+    fts.stepforward(prob, d3, 5)
+    assert calls == [1, 3, 1] and list(d3["steps"])[-2:] == [21, 24]
+    # the default storage (ceil((nsteps+1)/freq)) grows as the reference's extend! does
+    assert len(d3.t) == 14 and d3.i == 9

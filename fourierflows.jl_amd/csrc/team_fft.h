@@ -63,6 +63,24 @@ template <int N, int P> struct TeamSched {
 
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 
+#ifndef FF_LDS_BANKS
+#define FF_LDS_BANKS 32  // 4-byte LDS banks the slice rotation targets (0: no rotation)
+#endif
+
+// Complex slots between the LDS slices of k teams whose threads are
+// interleaved lane by lane (thread t belongs to team t % k).  Lanes of one
+// wavefront then touch the same slot index in k different slices; a stride
+// of slots + pad rotates slice c by c * BANKS / k banks, so those accesses
+// land on different banks instead of serialising on one.
+constexpr int team_slice(int slots, int slot_bytes, int k) {
+  if (k <= 1 || FF_LDS_BANKS == 0) return slots;
+  const int dw = slot_bytes / 4;
+  const int target = (FF_LDS_BANKS / k) % FF_LDS_BANKS;
+  for (int s = slots; s < slots + FF_LDS_BANKS; ++s)
+    if ((s * dw) % FF_LDS_BANKS == target) return s;
+  return slots;
+}
+
 template <int N, int P, int RK, int NS, bool FIRST, bool LAST, int DIR, typename R>
 __device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__ lds, int tid,
                                           const cplx<R>* __restrict__ tw) {

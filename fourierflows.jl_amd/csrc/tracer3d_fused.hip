@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(kVCB*(NZ / P)) k_tz(TracerGeom g, TracerArgs a
   const int t = threadIdx.x;
   const int c = t % CB;
   const int tid = t / CB;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * TeamSched<NZ, P>::LDS;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * team_slice(TeamSched<NZ, P>::LDS, sizeof(cplx<R>), CB);
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.twz);
   const int ix = tx * CB + c;
   const bool colkept = ix < g.KX;
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(kVCB*(NY / P)) k_ty(TracerGeom g, TracerYArgs 
   const int t = threadIdx.x;
   const int c = t % CB;
   const int tid = t / CB;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * TeamSched<NY, P>::LDS;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), CB);
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
   const int64_t base = ((int64_t)iz * g.TX + tx) * NY * CB + c;  // [z][tx][y][c]
 #pragma unroll 1
@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(4 * (NX / P)) k_tx(TracerGeom g, TracerArgs a)
   const int r = (int)(threadIdx.x % 4);
   const int y = (grp % (g.ny / 4)) * 4 + r;
   const int tid = threadIdx.x / 4;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + r * TeamSched<NX, P>::LDS;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + r * team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), 4);
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.twx);
   const int KX = g.KX;
   const int64_t ny = g.ny;
@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(kVCB*(NZ / P)) k_taz(TracerGeom g, TracerAdjAr
   const int t = threadIdx.x;
   const int c = t % CB;
   const int tid = t / CB;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * TeamSched<NZ, P>::LDS;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * team_slice(TeamSched<NZ, P>::LDS, sizeof(cplx<R>), CB);
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.twz);
   const int ix = tx * CB + c;
   const bool colkept = ix < g.KX;
@@ -443,7 +443,7 @@ __global__ void __launch_bounds__(4 * (NX / P)) k_tax(TracerGeom g, const void* 
   const int r = (int)(threadIdx.x % 4);
   const int y = (grp % (g.ny / 4)) * 4 + r;
   const int tid = threadIdx.x / 4;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + r * TeamSched<NX, P>::LDS;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + r * team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), 4);
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(twx);
   const int KX = g.KX;
   const int64_t ny = g.ny;
@@ -621,7 +621,7 @@ static bool tpow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 template <typename R, int N>
 static size_t t_lds(int teams) {
-  return (size_t)teams * TeamSched<N, vpts_col(N)>::LDS * sizeof(cplx<R>);
+  return (size_t)teams * team_slice(TeamSched<N, vpts_col(N)>::LDS, sizeof(cplx<R>), teams) * sizeof(cplx<R>);
 }
 
 template <typename K>
@@ -678,7 +678,7 @@ template <typename R, int N>
 static int launch_tx(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
   constexpr int P = vpts_row(N);
   auto kern = k_tx<R, N, P>;
-  const size_t sh = (size_t)4 * TeamSched<N, P>::LDS * sizeof(cplx<R>);
+  const size_t sh = (size_t)4 * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), 4) * sizeof(cplx<R>);
   static bool set = false;
   if (!set) {
     FF_TRY(t_attr(kern, sh));
@@ -794,7 +794,7 @@ template <typename R, int N>
 static int launch_tax(const TracerGeom& g, TracerState* s, hipStream_t st) {
   constexpr int P = vpts_row(N);
   auto kern = k_tax<R, N, P>;
-  const size_t sh = (size_t)4 * TeamSched<N, P>::LDS * sizeof(cplx<R>);
+  const size_t sh = (size_t)4 * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), 4) * sizeof(cplx<R>);
   static bool set = false;
   if (!set) {
     FF_TRY(t_attr(kern, sh));

@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vacol(VortGeom g, VortAdjColA
   const int cc = t % kCBK;
   const int c = sub * kCBK + cc;
   const int tid = t / kCBK;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + cc * TeamSched<NY, P>::LDS;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + cc * team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), kCBK);
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
   const int ix = tx * CB + c;
   const bool colkept = ix < g.KX;
@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_varow(VortGeom g, VortAdjRowA
   const int t = threadIdx.x;
   const int r = t % kRRK;
   const int tid = t / kRRK;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + r * TeamSched<NX, P>::LDS;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + r * team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), kRRK);
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
   const int y = grp * 4 + sub * kRRK + r;
   const int64_t ny = g.ny;
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_varow(VortGeom g, VortAdjRowA
 template <typename R, int N, int STAGE>
 static int launch_acol(const VortGeom& g, const VortAdjColArgs& a, hipStream_t st) {
   constexpr int P = vpts_col(N);
-  const size_t sh = (size_t)kCBK * TeamSched<N, P>::LDS * sizeof(cplx<R>);
+  const size_t sh = (size_t)kCBK * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), kCBK) * sizeof(cplx<R>);
   auto kern = k_vacol<R, N, P, STAGE>;
   static bool attr_set = false;
   if (!attr_set) {
@@ -319,7 +319,7 @@ static int launch_acol(const VortGeom& g, const VortAdjColArgs& a, hipStream_t s
 template <typename R, int N>
 static int launch_arow(const VortGeom& g, const VortAdjRowArgs& a, hipStream_t st) {
   constexpr int P = vpts_row(N);
-  const size_t sh = (size_t)kRRK * TeamSched<N, P>::LDS * sizeof(cplx<R>);
+  const size_t sh = (size_t)kRRK * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), kRRK) * sizeof(cplx<R>);
   auto kern = k_varow<R, N, P>;
   static bool attr_set = false;
   if (!attr_set) {

@@ -33,18 +33,24 @@
 namespace ffhip {
 
 
-// STAGE 0: y-inverse of U only.  STAGE 1..4: ETDRK4 stages.
+#ifndef FF_ROW_WAVES
+#define FF_ROW_WAVES 1  // minimum FP32 waves per SIMD the single-team row pass is compiled for
+#endif
+#ifndef FF_COL_WAVES
+#define FF_COL_WAVES 1  // same for the single-team column pass
+#endif
+
 // STAGE 0: y-inverse of U only.  STAGE 1..4: ETDRK4 stages.  kCBK columns per
 // workgroup; the kCSUB workgroups of one 4-column tile run on the same XCD.
 template <typename R, int NY, int P, int STAGE>
-__global__ void __launch_bounds__(kCBK*(NY / P)) k_vcol(VortGeom g, VortColArgs a) {
+__global__ void __launch_bounds__(kCBK*(NY / P), sizeof(R) == 4 ? FF_COL_WAVES : 1) k_vcol(VortGeom g, VortColArgs a) {
   extern __shared__ unsigned char vsm[];
   int tx, sub;
   xcd_split(blockIdx.x, kCSUB, &tx, &sub);
   if (tx >= g.TX) return;  // grid is rounded up to whole XCD groups
   const int t = threadIdx.x;
   const int cc = t % kCBK;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + cc * TeamSched<NY, P>::LDS;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + cc * team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), kCBK);
   vcol_body<R, NY, P, STAGE>(g, a, tx, sub * kCBK + cc, t / kCBK, lds, true);
 }
 
@@ -80,14 +86,14 @@ __global__ void __launch_bounds__(3 * (NX / P), sizeof(R) == 4 ? FF_ROW3_WAVES :
 
 // kRRK rows per workgroup; the kRSUB workgroups of one 4-row group run on the same XCD.
 template <typename R, int NX, int P>
-__global__ void __launch_bounds__(kRRK*(NX / P)) k_vrow(VortGeom g, VortRowArgs a) {
+__global__ void __launch_bounds__(kRRK*(NX / P), sizeof(R) == 4 ? FF_ROW_WAVES : 1) k_vrow(VortGeom g, VortRowArgs a) {
   extern __shared__ unsigned char vsm[];
   int grp, sub;
   xcd_split(blockIdx.x, kRSUB, &grp, &sub);
   if (grp >= g.ny / 4) return;  // grid is rounded up to whole XCD groups
   const int t = threadIdx.x;
   const int r = t % kRRK;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + r * TeamSched<NX, P>::LDS;
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + r * team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), kRRK);
   vrow_body<R, NX, P>(g, a, grp * 4 + sub * kRRK + r, t / kRRK, lds);
 }
 
@@ -96,12 +102,12 @@ static bool pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 template <typename R, int N>
 static size_t col_lds() {
-  return (size_t)kCBK * TeamSched<N, vpts_col(N)>::LDS * sizeof(cplx<R>);
+  return (size_t)kCBK * team_slice(TeamSched<N, vpts_col(N)>::LDS, sizeof(cplx<R>), kCBK) * sizeof(cplx<R>);
 }
 
 template <typename R, int N>
 static size_t row_lds() {
-  return (size_t)kRRK * TeamSched<N, vpts_row(N)>::LDS * sizeof(cplx<R>);
+  return (size_t)kRRK * team_slice(TeamSched<N, vpts_row(N)>::LDS, sizeof(cplx<R>), kRRK) * sizeof(cplx<R>);
 }
 
 template <typename R, int N, int STAGE>

@@ -8,6 +8,12 @@
 // complex values (one pad slot per 16 keeps strided writes off a single
 // bank).  The caller's block runs all teams through the same sequence, so
 // __syncthreads() is the team barrier.
+//
+// Twiddles come from an LDS table in pass order (team_tw_load): pass S's
+// factors for r = 1..RK-1 and k = 0..NS-1 sit at tw_off(S) + (r-1) NS + k,
+// so consecutive lanes read consecutive slots.  Reading them from global
+// memory instead put an L2 round trip into every pass and, since stores and
+// loads share one counter, made each pass wait for the stores before it.
 #pragma once
 
 #include "fft_device.h"
@@ -81,7 +87,58 @@ constexpr int team_slice(int slots, int slot_bytes, int k) {
   return slots;
 }
 
-template <int N, int P, int RK, int NS, bool FIRST, bool LAST, int DIR, typename R>
+// LDS twiddle table: slots before pass S's factors, and the table size
+template <int N, int P, int S>
+constexpr int tw_off() {
+  if constexpr (S <= 1) {
+    return 0;
+  } else {
+    constexpr int RK = S - 1 < TeamSched<N, P>::NFULL ? P : TeamSched<N, P>::REM;
+    return tw_off<N, P, S - 1>() + (RK - 1) * cpow(P, S - 1);
+  }
+}
+template <int N, int P>
+constexpr int tw_slots() {
+  return tw_off<N, P, TeamSched<N, P>::NPASS>();
+}
+// the same count for sizes known at run time (power-of-two n, points p)
+inline int tw_slots_rt(int n, int p) {
+  int total = 0, ns = 1, rest = n;
+  while (rest % p == 0 && rest >= p) {
+    if (ns > 1) total += (p - 1) * ns;
+    ns *= p;
+    rest /= p;
+  }
+  if (rest > 1 && ns > 1) total += (rest - 1) * ns;
+  return total;
+}
+
+template <int N, int P, int S, typename R>
+__device__ __forceinline__ void tw_fill_pass(cplx<R>* ltw, const cplx<R>* roots, int t, int nt) {
+  using TS = TeamSched<N, P>;
+  if constexpr (S < TS::NPASS) {
+    constexpr int RK = S < TS::NFULL ? P : TS::REM;
+    constexpr int NS = cpow(P, S);
+    constexpr int STEP = N / (NS * RK);
+    constexpr int OFF = tw_off<N, P, S>();
+    for (int e = t; e < (RK - 1) * NS; e += nt) {
+      const int r = e / NS + 1, k = e - (r - 1) * NS;
+      ltw[OFF + e] = roots[r * k * STEP];
+    }
+    tw_fill_pass<N, P, S + 1, R>(ltw, roots, t, nt);
+  }
+}
+
+// Fill the pass-ordered table from the N roots of unity (all threads of the
+// block; the caller's barrier publishes it).  Returns ltw.
+template <int N, int P, typename R>
+__device__ __forceinline__ const cplx<R>* team_tw_load(cplx<R>* ltw, const void* roots) {
+  tw_fill_pass<N, P, 1, R>(ltw, reinterpret_cast<const cplx<R>*>(roots), (int)threadIdx.x, (int)blockDim.x);
+  __syncthreads();
+  return ltw;
+}
+
+template <int N, int P, int RK, int NS, int TWO, bool FIRST, bool LAST, int DIR, typename R>
 __device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__ lds, int tid,
                                           const cplx<R>* __restrict__ tw) {
   constexpr int T = N / P;
@@ -106,9 +163,8 @@ __device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__
     const int k = j & (NS - 1);
     cplx<R>* y = x + b * RK;
     if constexpr (NS > 1) {
-      constexpr int TS = N / (NS * RK);
 #pragma unroll
-      for (int r = 1; r < RK; ++r) y[r] = cmul<R>(y[r], twid<DIR, R>(tw[r * k * TS]));
+      for (int r = 1; r < RK; ++r) y[r] = cmul<R>(y[r], twid<DIR, R>(tw[TWO + (r - 1) * NS + k]));
     }
     Bfly<RK, DIR, R>::run(y);
     if constexpr (LAST) {
@@ -129,13 +185,14 @@ __device__ __forceinline__ void team_passes(cplx<R> (&v)[P], cplx<R>* lds, int t
   if constexpr (S < TS::NPASS) {
     constexpr int RK = S < TS::NFULL ? P : TS::REM;
     constexpr int NS = cpow(P, S);
-    team_pass<N, P, RK, NS, S == 0, S == TS::NPASS - 1, DIR, R>(v, lds, tid, tw);
+    team_pass<N, P, RK, NS, tw_off<N, P, S>(), S == 0, S == TS::NPASS - 1, DIR, R>(v, lds, tid, tw);
     team_passes<N, P, S + 1, DIR, R>(v, lds, tid, tw);
   }
 }
 
 // Full transform.  Requires NPASS >= 2 (N >= 2P) so that the first pass
 // writes LDS and the last pass reads it.  Ends with LDS free for reuse.
+// tw is the table team_tw_load filled.
 template <int N, int P, int DIR, typename R>
 __device__ __forceinline__ void team_fft(cplx<R> (&v)[P], cplx<R>* lds, int tid, const cplx<R>* tw) {
   static_assert(TeamSched<N, P>::NPASS >= 2, "team_fft needs at least two passes");

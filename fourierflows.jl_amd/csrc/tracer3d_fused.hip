@@ -91,8 +91,9 @@ __global__ void __launch_bounds__(kVCB*(NZ / P)) k_tz(TracerGeom g, TracerArgs a
   const int t = threadIdx.x;
   const int c = t % CB;
   const int tid = t / CB;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * team_slice(TeamSched<NZ, P>::LDS, sizeof(cplx<R>), CB);
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.twz);
+  constexpr int SL = team_slice(TeamSched<NZ, P>::LDS, sizeof(cplx<R>), CB);
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * SL;
+  const cplx<R>* tw = team_tw_load<NZ, P, R>(reinterpret_cast<cplx<R>*>(tsm) + CB * SL, a.twz);
   const int ix = tx * CB + c;
   const bool colkept = ix < g.KX;
   const int iy = t_full_index(ky, g.ylo, g.yhi);
@@ -170,7 +171,7 @@ __global__ void __launch_bounds__(kVCB*(NZ / P)) k_tz(TracerGeom g, TracerArgs a
     }
   }
   if (!a.do_inverse) return;  // uniform over the block
-  team_fft<NZ, P, 1, R>(nw, lds, tid, reinterpret_cast<const cplx<R>*>(a.twz));
+  team_fft<NZ, P, 1, R>(nw, lds, tid, tw);
   cplx<R>* A1 = reinterpret_cast<cplx<R>*>(a.A1);
   const int64_t plane = (int64_t)g.TX * g.ny * CB;  // one z plane of A1
   const int64_t abase = ((int64_t)tx * g.ny + iy) * CB + c;
@@ -200,8 +201,9 @@ __global__ void __launch_bounds__(kVCB*(NY / P)) k_ty(TracerGeom g, TracerYArgs 
   const int t = threadIdx.x;
   const int c = t % CB;
   const int tid = t / CB;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), CB);
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
+  constexpr int SL = team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), CB);
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * SL;
+  const cplx<R>* tw = team_tw_load<NY, P, R>(reinterpret_cast<cplx<R>*>(tsm) + CB * SL, a.tw);
   const int64_t base = ((int64_t)iz * g.TX + tx) * NY * CB + c;  // [z][tx][y][c]
 #pragma unroll 1
   for (int f = 0; f < NF; ++f) {
@@ -239,8 +241,9 @@ __global__ void __launch_bounds__(4 * (NX / P)) k_tx(TracerGeom g, TracerArgs a)
   const int r = (int)(threadIdx.x % 4);
   const int y = (grp % (g.ny / 4)) * 4 + r;
   const int tid = threadIdx.x / 4;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + r * team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), 4);
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.twx);
+  constexpr int SL = team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), 4);
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + r * SL;
+  const cplx<R>* tw = team_tw_load<NX, P, R>(reinterpret_cast<cplx<R>*>(tsm) + 4 * SL, a.twx);
   const int KX = g.KX;
   const int64_t ny = g.ny;
   const int64_t zoff = (int64_t)iz * g.TX * ny * CB;
@@ -357,8 +360,9 @@ __global__ void __launch_bounds__(kVCB*(NZ / P)) k_taz(TracerGeom g, TracerAdjAr
   const int t = threadIdx.x;
   const int c = t % CB;
   const int tid = t / CB;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * team_slice(TeamSched<NZ, P>::LDS, sizeof(cplx<R>), CB);
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.twz);
+  constexpr int SL = team_slice(TeamSched<NZ, P>::LDS, sizeof(cplx<R>), CB);
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + c * SL;
+  const cplx<R>* tw = team_tw_load<NZ, P, R>(reinterpret_cast<cplx<R>*>(tsm) + CB * SL, a.twz);
   const int ix = tx * CB + c;
   const bool colkept = ix < g.KX;
   const int iy = t_full_index(ky, g.ylo, g.yhi);
@@ -443,8 +447,9 @@ __global__ void __launch_bounds__(4 * (NX / P)) k_tax(TracerGeom g, const void* 
   const int r = (int)(threadIdx.x % 4);
   const int y = (grp % (g.ny / 4)) * 4 + r;
   const int tid = threadIdx.x / 4;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + r * team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), 4);
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(twx);
+  constexpr int SL = team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), 4);
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(tsm) + r * SL;
+  const cplx<R>* tw = team_tw_load<NX, P, R>(reinterpret_cast<cplx<R>*>(tsm) + 4 * SL, twx);
   const int KX = g.KX;
   const int64_t ny = g.ny;
   const int64_t zoff = (int64_t)iz * g.TX * ny * CB;
@@ -621,7 +626,8 @@ static bool tpow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 template <typename R, int N>
 static size_t t_lds(int teams) {
-  return (size_t)teams * team_slice(TeamSched<N, vpts_col(N)>::LDS, sizeof(cplx<R>), teams) * sizeof(cplx<R>);
+  return ((size_t)teams * team_slice(TeamSched<N, vpts_col(N)>::LDS, sizeof(cplx<R>), teams) +
+          tw_slots<N, vpts_col(N)>()) * sizeof(cplx<R>);
 }
 
 template <typename K>
@@ -678,7 +684,7 @@ template <typename R, int N>
 static int launch_tx(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
   constexpr int P = vpts_row(N);
   auto kern = k_tx<R, N, P>;
-  const size_t sh = (size_t)4 * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), 4) * sizeof(cplx<R>);
+  const size_t sh = ((size_t)4 * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), 4) + tw_slots<N, P>()) * sizeof(cplx<R>);
   static bool set = false;
   if (!set) {
     FF_TRY(t_attr(kern, sh));
@@ -794,7 +800,7 @@ template <typename R, int N>
 static int launch_tax(const TracerGeom& g, TracerState* s, hipStream_t st) {
   constexpr int P = vpts_row(N);
   auto kern = k_tax<R, N, P>;
-  const size_t sh = (size_t)4 * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), 4) * sizeof(cplx<R>);
+  const size_t sh = ((size_t)4 * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), 4) + tw_slots<N, P>()) * sizeof(cplx<R>);
   static bool set = false;
   if (!set) {
     FF_TRY(t_attr(kern, sh));

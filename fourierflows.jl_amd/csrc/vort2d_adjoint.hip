@@ -77,8 +77,9 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vacol(VortGeom g, VortAdjColA
   const int cc = t % kCBK;
   const int c = sub * kCBK + cc;
   const int tid = t / kCBK;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + cc * team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), kCBK);
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
+  constexpr int SL = team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), kCBK);
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + cc * SL;
+  const cplx<R>* tw = team_tw_load<NY, P, R>(reinterpret_cast<cplx<R>*>(vsm) + kCBK * SL, a.tw);
   const int ix = tx * CB + c;
   const bool colkept = ix < g.KX;
   const R kx = (R)(g.dk * (double)ix);
@@ -213,8 +214,9 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_varow(VortGeom g, VortAdjRowA
   const int t = threadIdx.x;
   const int r = t % kRRK;
   const int tid = t / kRRK;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + r * team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), kRRK);
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
+  constexpr int SL = team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), kRRK);
+  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + r * SL;
+  const cplx<R>* tw = team_tw_load<NX, P, R>(reinterpret_cast<cplx<R>*>(vsm) + kRRK * SL, a.tw);
   const int y = grp * 4 + sub * kRRK + r;
   const int64_t ny = g.ny;
   auto iaddr = [&](int k) -> uint32_t { return ((uint32_t)(k / CB) * (uint32_t)ny + y) * CB + (k % CB); };
@@ -292,7 +294,8 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_varow(VortGeom g, VortAdjRowA
 template <typename R, int N, int STAGE>
 static int launch_acol(const VortGeom& g, const VortAdjColArgs& a, hipStream_t st) {
   constexpr int P = vpts_col(N);
-  const size_t sh = (size_t)kCBK * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), kCBK) * sizeof(cplx<R>);
+  const size_t sh =
+      ((size_t)kCBK * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), kCBK) + tw_slots<N, P>()) * sizeof(cplx<R>);
   auto kern = k_vacol<R, N, P, STAGE>;
   static bool attr_set = false;
   if (!attr_set) {
@@ -319,7 +322,8 @@ static int launch_acol(const VortGeom& g, const VortAdjColArgs& a, hipStream_t s
 template <typename R, int N>
 static int launch_arow(const VortGeom& g, const VortAdjRowArgs& a, hipStream_t st) {
   constexpr int P = vpts_row(N);
-  const size_t sh = (size_t)kRRK * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), kRRK) * sizeof(cplx<R>);
+  const size_t sh =
+      ((size_t)kRRK * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), kRRK) + tw_slots<N, P>()) * sizeof(cplx<R>);
   auto kern = k_varow<R, N, P>;
   static bool attr_set = false;
   if (!attr_set) {

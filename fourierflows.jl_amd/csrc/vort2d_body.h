@@ -84,10 +84,9 @@ __device__ __forceinline__ cplx<R> vfield(int f, const cplx<R> z, const R kx, co
 // zeros and writes nothing.
 template <typename R, int NY, int P, int STAGE>
 __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& a, const int tx, const int c,
-                                          const int tid, cplx<R>* lds, const bool live) {
+                                          const int tid, cplx<R>* lds, const cplx<R>* tw, const bool live) {
   constexpr int T = NY / P;
   constexpr int CB = kVCB;
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
   const int ix = tx * CB + c;
   const bool colkept = live && ix < g.KX;
   const R kx = (R)(g.dk * (double)ix);
@@ -161,13 +160,12 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
 // instead of five.
 template <typename R, int NY, int P, int STAGE>
 __device__ __forceinline__ void vcol3_body(const VortGeom& g, const VortColArgs& a, const int tx, const int c,
-                                           const int team, const int tid, cplx<R>* lds) {
+                                           const int team, const int tid, cplx<R>* lds, const cplx<R>* tw) {
   constexpr int T = NY / P;
   constexpr int CB = kVCB;
   constexpr int L = TeamSched<NY, P>::LDS;
   constexpr int NBAR = 2 * (TeamSched<NY, P>::NPASS - 1);  // barriers inside one team_fft
   static_assert(T % 64 == 0, "three-team column pass needs whole wavefronts per team");
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
   cplx<R>* mine = lds + team * L;
   cplx<R>* RA = lds;          // forward (u zeta)^ after the y-FFT, natural order
   cplx<R>* RB = lds + L;      // forward (v zeta)^
@@ -232,10 +230,9 @@ __device__ __forceinline__ void vcol3_body(const VortGeom& g, const VortColArgs&
 // tid) with its own LDS slice.
 template <typename R, int NX, int P>
 __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& a, const int y, const int tid,
-                                          cplx<R>* lds) {
+                                          cplx<R>* lds, const cplx<R>* tw) {
   constexpr int T = NX / P;
   constexpr int CB = kVCB;
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
   const int64_t ny = g.ny;
   auto iaddr = [&](int k) -> int64_t { return ((int64_t)(k / CB) * ny + y) * CB + (k % CB); };
   const cplx<R>* Zh = reinterpret_cast<const cplx<R>*>(a.I0);
@@ -334,13 +331,12 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
 // rows instead of six, two in the serial chain instead of three.
 template <typename R, int NX, int P>
 __device__ __forceinline__ void vrow3_body(const VortGeom& g, const VortRowArgs& a, const int y0, const int team,
-                                           const int tid, cplx<R>* lds) {
+                                           const int tid, cplx<R>* lds, const cplx<R>* tw) {
   constexpr int T = NX / P;
   constexpr int CB = kVCB;
   constexpr int L = TeamSched<NX, P>::LDS;
   constexpr int NBAR = 2 * (TeamSched<NX, P>::NPASS - 1);  // barriers inside one team_fft
   static_assert(T % 64 == 0, "three-team row pass needs whole wavefronts per team");
-  const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.tw);
   const int64_t ny = g.ny;
   const int KX = g.KX;
   cplx<R>* mine = lds + team * L;

@@ -45,8 +45,10 @@ __device__ __forceinline__ void ens_col_n(const VortGeom& g, const VortColArgs& 
     const int t = threadIdx.x;
     const int cc = t % CPB;
     const int col = first + cc;
-    cplx<R>* lds = reinterpret_cast<cplx<R>*>(sm) + cc * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), CPB);
-    vcol_body<R, N, P, STAGE>(g, a, col / kVCB, col % kVCB, t / CPB, lds, col / kVCB < g.TX);
+    constexpr int SL = team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), CPB);
+    cplx<R>* base = reinterpret_cast<cplx<R>*>(sm);
+    const cplx<R>* tw = team_tw_load<N, P, R>(base + CPB * SL, a.tw);
+    vcol_body<R, N, P, STAGE>(g, a, col / kVCB, col % kVCB, t / CPB, base + cc * SL, tw, col / kVCB < g.TX);
   }
 }
 
@@ -58,8 +60,10 @@ __device__ __forceinline__ void ens_row_n(const VortGeom& g, const VortRowArgs& 
     constexpr int RPB = BD / T;
     const int t = threadIdx.x;
     const int r = t % RPB;
-    cplx<R>* lds = reinterpret_cast<cplx<R>*>(sm) + r * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), RPB);
-    vrow_body<R, N, P>(g, a, first + r, t / RPB, lds);
+    constexpr int SL = team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), RPB);
+    cplx<R>* base = reinterpret_cast<cplx<R>*>(sm);
+    const cplx<R>* tw = team_tw_load<N, P, R>(base + RPB * SL, a.tw);
+    vrow_body<R, N, P>(g, a, first + r, t / RPB, base + r * SL, tw);
   }
 }
 
@@ -151,8 +155,8 @@ static int build_items(ffhip_ensemble* e, EnsVariant v) {
              v.prule, g.nx, g.ny);
     for (int c = 0; c < g.TX * kVCB; c += cpb) cols.push_back(make_int2(m, c));
     for (int y = 0; y < g.ny; y += rpb) rows.push_back(make_int2(m, y));
-    lc = std::max(lc, (size_t)cpb * (size_t)team_slice(g.ny + g.ny / 16, (int)cs, cpb) * cs);
-    lr = std::max(lr, (size_t)rpb * (size_t)team_slice(g.nx + g.nx / 16, (int)cs, rpb) * cs);
+    lc = std::max(lc, ((size_t)cpb * (size_t)team_slice(g.ny + g.ny / 16, (int)cs, cpb) + tw_slots_rt(g.ny, Py)) * cs);
+    lr = std::max(lr, ((size_t)rpb * (size_t)team_slice(g.nx + g.nx / 16, (int)cs, rpb) + tw_slots_rt(g.nx, Px)) * cs);
   }
   FF_CHECK(lc <= (size_t)max_lds_per_block() && lr <= (size_t)max_lds_per_block(), FFHIP_ERR_UNSUPPORTED,
            "ensemble variant needs more LDS than a workgroup has");

@@ -50,8 +50,10 @@ __global__ void __launch_bounds__(kCBK*(NY / P), sizeof(R) == 4 ? FF_COL_WAVES :
   if (tx >= g.TX) return;  // grid is rounded up to whole XCD groups
   const int t = threadIdx.x;
   const int cc = t % kCBK;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + cc * team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), kCBK);
-  vcol_body<R, NY, P, STAGE>(g, a, tx, sub * kCBK + cc, t / kCBK, lds, true);
+  constexpr int SL = team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), kCBK);
+  cplx<R>* base = reinterpret_cast<cplx<R>*>(vsm);
+  const cplx<R>* tw = team_tw_load<NY, P, R>(base + kCBK * SL, a.tw);
+  vcol_body<R, NY, P, STAGE>(g, a, tx, sub * kCBK + cc, t / kCBK, base + cc * SL, tw, true);
 }
 
 // One column per workgroup of three teams (vcol3_body); the four columns of a
@@ -64,7 +66,9 @@ __global__ void __launch_bounds__(3 * (NY / P)) k_vcol3(VortGeom g, VortColArgs 
   if (tx >= g.TX) return;  // grid is rounded up to whole XCD groups
   constexpr int T = NY / P;
   const int t = threadIdx.x;
-  vcol3_body<R, NY, P, STAGE>(g, a, tx, c, t / T, t % T, reinterpret_cast<cplx<R>*>(vsm));
+  cplx<R>* base = reinterpret_cast<cplx<R>*>(vsm);
+  const cplx<R>* tw = team_tw_load<NY, P, R>(base + 3 * TeamSched<NY, P>::LDS, a.tw);
+  vcol3_body<R, NY, P, STAGE>(g, a, tx, c, t / T, t % T, base, tw);
 }
 
 // Two rows per workgroup of three teams (vrow3_body); the two workgroups of
@@ -81,7 +85,9 @@ __global__ void __launch_bounds__(3 * (NX / P), sizeof(R) == 4 ? FF_ROW3_WAVES :
   if (grp >= g.ny / 4) return;  // grid is rounded up to whole XCD groups
   constexpr int T = NX / P;
   const int t = threadIdx.x;
-  vrow3_body<R, NX, P>(g, a, grp * 4 + sub * 2, t / T, t % T, reinterpret_cast<cplx<R>*>(vsm));
+  cplx<R>* base = reinterpret_cast<cplx<R>*>(vsm);
+  const cplx<R>* tw = team_tw_load<NX, P, R>(base + 3 * TeamSched<NX, P>::LDS, a.tw);
+  vrow3_body<R, NX, P>(g, a, grp * 4 + sub * 2, t / T, t % T, base, tw);
 }
 
 // kRRK rows per workgroup; the kRSUB workgroups of one 4-row group run on the same XCD.
@@ -93,8 +99,10 @@ __global__ void __launch_bounds__(kRRK*(NX / P), sizeof(R) == 4 ? FF_ROW_WAVES :
   if (grp >= g.ny / 4) return;  // grid is rounded up to whole XCD groups
   const int t = threadIdx.x;
   const int r = t % kRRK;
-  cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + r * team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), kRRK);
-  vrow_body<R, NX, P>(g, a, grp * 4 + sub * kRRK + r, t / kRRK, lds);
+  constexpr int SL = team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), kRRK);
+  cplx<R>* base = reinterpret_cast<cplx<R>*>(vsm);
+  const cplx<R>* tw = team_tw_load<NX, P, R>(base + kRRK * SL, a.tw);
+  vrow_body<R, NX, P>(g, a, grp * 4 + sub * kRRK + r, t / kRRK, base + r * SL, tw);
 }
 
 // ----------------------------------------------------------------- host side
@@ -102,12 +110,14 @@ static bool pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 template <typename R, int N>
 static size_t col_lds() {
-  return (size_t)kCBK * team_slice(TeamSched<N, vpts_col(N)>::LDS, sizeof(cplx<R>), kCBK) * sizeof(cplx<R>);
+  return ((size_t)kCBK * team_slice(TeamSched<N, vpts_col(N)>::LDS, sizeof(cplx<R>), kCBK) +
+          tw_slots<N, vpts_col(N)>()) * sizeof(cplx<R>);
 }
 
 template <typename R, int N>
 static size_t row_lds() {
-  return (size_t)kRRK * team_slice(TeamSched<N, vpts_row(N)>::LDS, sizeof(cplx<R>), kRRK) * sizeof(cplx<R>);
+  return ((size_t)kRRK * team_slice(TeamSched<N, vpts_row(N)>::LDS, sizeof(cplx<R>), kRRK) +
+          tw_slots<N, vpts_row(N)>()) * sizeof(cplx<R>);
 }
 
 template <typename R, int N, int STAGE>
@@ -151,7 +161,7 @@ static bool row3_on(bool f32) {
 template <typename R, int N, int STAGE>
 static int launch_col3(const VortGeom& g, const VortColArgs& a, hipStream_t st) {
   constexpr int P = vpts_col(N);
-  const size_t sh = (size_t)3 * TeamSched<N, P>::LDS * sizeof(cplx<R>);
+  const size_t sh = ((size_t)3 * TeamSched<N, P>::LDS + tw_slots<N, P>()) * sizeof(cplx<R>);
   auto kern = k_vcol3<R, N, P, STAGE>;
   static bool attr_set = false;
   if (!attr_set) {
@@ -172,7 +182,7 @@ static int launch_col3(const VortGeom& g, const VortColArgs& a, hipStream_t st) 
 template <typename R, int N>
 static int launch_row3(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
   constexpr int P = vpts_row(N);
-  const size_t sh = (size_t)3 * TeamSched<N, P>::LDS * sizeof(cplx<R>);
+  const size_t sh = ((size_t)3 * TeamSched<N, P>::LDS + tw_slots<N, P>()) * sizeof(cplx<R>);
   auto kern = k_vrow3<R, N, P>;
   static bool attr_set = false;
   if (!attr_set) {

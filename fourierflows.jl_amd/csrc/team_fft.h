@@ -138,9 +138,52 @@ __device__ __forceinline__ const cplx<R>* team_tw_load(cplx<R>* ltw, const void*
   return ltw;
 }
 
-template <int N, int P, int RK, int NS, int TWO, bool FIRST, bool LAST, int DIR, typename R>
+// Per-thread twiddle cache: a thread's factors of every pass in registers
+// (pass S, butterfly b, r at twr_off(S) + b (RK-1) + r-1), for kernels that
+// run several transforms of one size and have the registers to spare.
+template <int N, int P, int S>
+constexpr int twr_off() {
+  if constexpr (S <= 1) {
+    return 0;
+  } else {
+    constexpr int RK = S - 1 < TeamSched<N, P>::NFULL ? P : TeamSched<N, P>::REM;
+    return twr_off<N, P, S - 1>() + (P / RK) * (RK - 1);
+  }
+}
+// NREG: passes 1..NREG keep their factors in registers, later passes read LDS
+template <int N, int P, typename R, int NREG = TeamSched<N, P>::NPASS - 1>
+struct TwRegs {
+  static constexpr int NR = NREG;
+  cplx<R> w[twr_off<N, P, NREG + 1>()];
+};
+
+template <int N, int P, int S, typename R, int NREG>
+__device__ __forceinline__ void twr_fill(TwRegs<N, P, R, NREG>& t, const cplx<R>* ltw, int tid) {
+  using TS = TeamSched<N, P>;
+  if constexpr (S < TS::NPASS && S <= NREG) {
+    constexpr int RK = S < TS::NFULL ? P : TS::REM;
+    constexpr int NS = cpow(P, S);
+    constexpr int NB = P / RK;
+    constexpr int T = N / P;
+    constexpr int OFF = tw_off<N, P, S>();
+    constexpr int RO = twr_off<N, P, S>();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int k = (tid + b * T) & (NS - 1);
+#pragma unroll
+      for (int r = 1; r < RK; ++r) t.w[RO + b * (RK - 1) + r - 1] = ltw[OFF + (r - 1) * NS + k];
+    }
+    twr_fill<N, P, S + 1, R, NREG>(t, ltw, tid);
+  }
+}
+template <int N, int P, typename R, int NREG>
+__device__ __forceinline__ void twr_fill(TwRegs<N, P, R, NREG>& t, const cplx<R>* ltw, int tid) {
+  twr_fill<N, P, 1, R, NREG>(t, ltw, tid);
+}
+
+template <int N, int P, int RK, int NS, int TWO, int TRO, bool REG, bool FIRST, bool LAST, int DIR, typename R>
 __device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__ lds, int tid,
-                                          const cplx<R>* __restrict__ tw) {
+                                          const cplx<R>* __restrict__ tw, const cplx<R>* twr) {
   constexpr int T = N / P;
   constexpr int NB = P / RK;  // butterflies per thread
   constexpr int M = N / RK;
@@ -164,7 +207,10 @@ __device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__
     cplx<R>* y = x + b * RK;
     if constexpr (NS > 1) {
 #pragma unroll
-      for (int r = 1; r < RK; ++r) y[r] = cmul<R>(y[r], twid<DIR, R>(tw[TWO + (r - 1) * NS + k]));
+      for (int r = 1; r < RK; ++r) {
+        const cplx<R> w = REG ? twr[TRO + b * (RK - 1) + r - 1] : tw[TWO + (r - 1) * NS + k];
+        y[r] = cmul<R>(y[r], twid<DIR, R>(w));
+      }
     }
     Bfly<RK, DIR, R>::run(y);
     if constexpr (LAST) {
@@ -179,14 +225,16 @@ __device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__
   if constexpr (!LAST) __syncthreads();
 }
 
-template <int N, int P, int S, int DIR, typename R>
-__device__ __forceinline__ void team_passes(cplx<R> (&v)[P], cplx<R>* lds, int tid, const cplx<R>* tw) {
+template <int N, int P, int S, int DIR, int NREG, typename R>
+__device__ __forceinline__ void team_passes(cplx<R> (&v)[P], cplx<R>* lds, int tid, const cplx<R>* tw,
+                                            const cplx<R>* twr) {
   using TS = TeamSched<N, P>;
   if constexpr (S < TS::NPASS) {
     constexpr int RK = S < TS::NFULL ? P : TS::REM;
     constexpr int NS = cpow(P, S);
-    team_pass<N, P, RK, NS, tw_off<N, P, S>(), S == 0, S == TS::NPASS - 1, DIR, R>(v, lds, tid, tw);
-    team_passes<N, P, S + 1, DIR, R>(v, lds, tid, tw);
+    team_pass<N, P, RK, NS, tw_off<N, P, S>(), twr_off<N, P, S>(), (S <= NREG), S == 0, S == TS::NPASS - 1, DIR,
+              R>(v, lds, tid, tw, twr);
+    team_passes<N, P, S + 1, DIR, NREG, R>(v, lds, tid, tw, twr);
   }
 }
 
@@ -196,7 +244,16 @@ __device__ __forceinline__ void team_passes(cplx<R> (&v)[P], cplx<R>* lds, int t
 template <int N, int P, int DIR, typename R>
 __device__ __forceinline__ void team_fft(cplx<R> (&v)[P], cplx<R>* lds, int tid, const cplx<R>* tw) {
   static_assert(TeamSched<N, P>::NPASS >= 2, "team_fft needs at least two passes");
-  team_passes<N, P, 0, DIR, R>(v, lds, tid, tw);
+  team_passes<N, P, 0, DIR, 0, R>(v, lds, tid, tw, nullptr);
+}
+
+// The same transform with the first t.NR passes' twiddles from registers
+// (twr_fill) and the rest from the LDS table tw.
+template <int N, int P, int DIR, typename R, int NREG>
+__device__ __forceinline__ void team_fft(cplx<R> (&v)[P], cplx<R>* lds, int tid, const cplx<R>* tw,
+                                         const TwRegs<N, P, R, NREG>& t) {
+  static_assert(TeamSched<N, P>::NPASS >= 2, "team_fft needs at least two passes");
+  team_passes<N, P, 0, DIR, NREG, R>(v, lds, tid, tw, t.w);
 }
 
 }  // namespace ffhip

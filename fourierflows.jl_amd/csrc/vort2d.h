@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <type_traits>
 
 #include "ops.h"
 #include "prof.h"
@@ -11,6 +12,12 @@ namespace ffhip {
 
 #ifndef FF_ROW_PREFETCH_F32
 #define FF_ROW_PREFETCH_F32 0
+#endif
+#ifndef FF_ROW_TWREG
+#define FF_ROW_TWREG 1  // FP32 row pass: twiddles in registers across its three transforms
+#endif
+#ifndef FF_COL_TWREG
+#define FF_COL_TWREG 1  // FP32 column pass: first-pass twiddles in registers across its five transforms
 #endif
 #ifndef FF_ROW_PREFETCH_F64
 #define FF_ROW_PREFETCH_F64 1

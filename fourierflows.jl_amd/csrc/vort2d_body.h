@@ -92,6 +92,19 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
   const R kx = (R)(g.dk * (double)ix);
   const int64_t ibase = (int64_t)tx * NY * CB + c;
   const int64_t sbase = (int64_t)tx * g.KY * CB + c;
+  // the five transforms share one set of twiddles: FP32 keeps the first
+  // pass's factors in registers (all passes would cost the second wavefront)
+  constexpr bool treg = sizeof(R) == 4 && FF_COL_TWREG;
+  TwRegs<NY, P, R, 1> twr;
+  if constexpr (treg) twr_fill(twr, tw, tid);
+  auto fft = [&](cplx<R>(&x)[P], auto dir) {
+    if constexpr (treg)
+      team_fft<NY, P, decltype(dir)::value, R>(x, lds, tid, tw, twr);
+    else
+      team_fft<NY, P, decltype(dir)::value, R>(x, lds, tid, tw);
+  };
+  using Inv = std::integral_constant<int, 1>;
+  using Fwd = std::integral_constant<int, -1>;
 
   cplx<R> nn[P];
   if constexpr (STAGE == 0) {
@@ -104,10 +117,10 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
     const cplx<R>* O1 = reinterpret_cast<const cplx<R>*>(a.O1);
 #pragma unroll
     for (int p = 0; p < P; ++p) nn[p] = live ? O0[ibase + (int64_t)(tid + p * T) * CB] : mkc<R>(0, 0);
-    team_fft<NY, P, -1, R>(nn, lds, tid, tw);
+    fft(nn, Fwd{});
 #pragma unroll
     for (int p = 0; p < P; ++p) vb[p] = live ? O1[ibase + (int64_t)(tid + p * T) * CB] : mkc<R>(0, 0);
-    team_fft<NY, P, -1, R>(vb, lds, tid, tw);
+    fft(vb, Fwd{});
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       const int iy = tid + p * T;
@@ -141,7 +154,7 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
       const int iy = tid + p * T;
       v[p] = vfield<R>(f, nw[p], kx, (R)(g.dl * (double)vfreq(iy, NY)), ix == 0 && iy == 0);
     }
-    team_fft<NY, P, 1, R>(v, lds, tid, tw);
+    fft(v, Inv{});
     cplx<R>* d = dst[f];
     if (live) {
 #pragma unroll
@@ -239,6 +252,18 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
   const cplx<R>* Uh = reinterpret_cast<const cplx<R>*>(a.I1);
   const cplx<R>* Vh = reinterpret_cast<const cplx<R>*>(a.I2);
   const int KX = g.KX;
+  // the three transforms share one set of twiddles: FP32 keeps it in registers
+  constexpr bool treg = sizeof(R) == 4 && FF_ROW_TWREG;
+  TwRegs<NX, P, R> twr;
+  if constexpr (treg) twr_fill(twr, tw, tid);
+  auto fft = [&](cplx<R>(&x)[P], auto dir) {
+    if constexpr (treg)
+      team_fft<NX, P, decltype(dir)::value, R>(x, lds, tid, tw, twr);
+    else
+      team_fft<NX, P, decltype(dir)::value, R>(x, lds, tid, tw);
+  };
+  using Inv = std::integral_constant<int, 1>;
+  using Fwd = std::integral_constant<int, -1>;
 
   // u + i v: Hermitian-extended half spectra packed into one complex inverse
   cplx<R> z[P];
@@ -260,9 +285,9 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
     z[p] = s;
   }
   // FP64: issue the zeta^ loads before the u + i v transform (measured faster);
-  // FP32: transform first (the extra live registers cost more than they hide)
+  // FP32: transform first (with the twiddles in registers, measured 3% faster)
   constexpr bool prefetch = sizeof(R) == 8 ? FF_ROW_PREFETCH_F64 : FF_ROW_PREFETCH_F32;
-  if constexpr (!prefetch) team_fft<NX, P, 1, R>(z, lds, tid, tw);
+  if constexpr (!prefetch) fft(z, Inv{});
 
   cplx<R> w[P];
 #pragma unroll
@@ -278,8 +303,8 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
     }
     w[p] = s;
   }
-  if constexpr (prefetch) team_fft<NX, P, 1, R>(z, lds, tid, tw);
-  team_fft<NX, P, 1, R>(w, lds, tid, tw);
+  if constexpr (prefetch) fft(z, Inv{});
+  fft(w, Inv{});
 
   // products u zeta + i v zeta with the 1/(nx ny)^2 normalisation
   const R s2 = (R)g.s2;
@@ -298,7 +323,7 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
     const R zz = w[p].x * s2;
     w[p] = mkc<R>(z[p].x * zz, z[p].y * zz);
   }
-  team_fft<NX, P, -1, R>(w, lds, tid, tw);
+  fft(w, Fwd{});
 
   // split W into the half spectra of its real and imaginary parts
 #pragma unroll

@@ -86,8 +86,15 @@ __global__ void __launch_bounds__(kVCB*(NZ / P)) k_tz(TracerGeom g, TracerArgs a
   constexpr int T = NZ / P;
   constexpr int CB = kVCB;
   extern __shared__ unsigned char tsm[];
-  const int grp = (int)blockIdx.x;  // one 4-column tile at one kept y per workgroup
-  const int tx = grp / g.KY, ky = grp % g.KY;
+  // one 4-column tile at one kept y per workgroup; the four y rows that share
+  // the 128 B lines of the [nz][TX][ny][4] output run on one XCD (one L2)
+  int q, sub;
+  xcd_split(blockIdx.x, 4, &q, &sub);
+  const int nq = g.ny / 4;
+  if (q >= g.TX * nq) return;
+  const int tx = q / nq, iy = (q % nq) * 4 + sub;
+  if (iy >= g.ylo && iy <= g.yhi) return;  // aliased row: no work
+  const int ky = t_kept_index(iy, g.ylo, g.yhi);
   const int t = threadIdx.x;
   const int c = t % CB;
   const int tid = t / CB;
@@ -96,7 +103,6 @@ __global__ void __launch_bounds__(kVCB*(NZ / P)) k_tz(TracerGeom g, TracerArgs a
   const cplx<R>* tw = team_tw_load<NZ, P, R>(reinterpret_cast<cplx<R>*>(tsm) + CB * SL, a.twz);
   const int ix = tx * CB + c;
   const bool colkept = ix < g.KX;
-  const int iy = t_full_index(ky, g.ylo, g.yhi);
   const R kx = (R)(g.dk * (double)ix);
   const R l = (R)(g.dl * (double)vfreq(iy, g.ny));
   const int64_t cbase = ((int64_t)tx * g.KY + ky) * NZ * CB + c;      // C[f][tx][ky][z][c]
@@ -196,8 +202,13 @@ __global__ void __launch_bounds__(kVCB*(NY / P)) k_ty(TracerGeom g, TracerYArgs 
   constexpr int T = NY / P;
   constexpr int CB = kVCB;
   extern __shared__ unsigned char tsm[];
-  const int grp = (int)blockIdx.x;  // one 4-column tile at one z per workgroup
-  const int iz = grp / g.TX, tx = grp % g.TX;
+  // one 4-column tile at one z per workgroup; the four z planes that share the
+  // 128 B lines of a [TX][KY][nz][4] output run on one XCD (one L2)
+  int q, sub;
+  xcd_split(blockIdx.x, 4, &q, &sub);
+  const int nq = g.nz / 4;
+  if (q >= g.TX * nq) return;
+  const int tx = q % g.TX, iz = (q / g.TX) * 4 + sub;
   const int t = threadIdx.x;
   const int c = t % CB;
   const int tid = t / CB;
@@ -355,8 +366,13 @@ __global__ void __launch_bounds__(kVCB*(NZ / P)) k_taz(TracerGeom g, TracerAdjAr
   constexpr int T = NZ / P;
   constexpr int CB = kVCB;
   extern __shared__ unsigned char tsm[];
-  const int grp = (int)blockIdx.x;
-  const int tx = grp / g.KY, ky = grp % g.KY;
+  int q, sub;  // work mapping as in k_tz
+  xcd_split(blockIdx.x, 4, &q, &sub);
+  const int nq = g.ny / 4;
+  if (q >= g.TX * nq) return;
+  const int tx = q / nq, iy = (q % nq) * 4 + sub;
+  if (iy >= g.ylo && iy <= g.yhi) return;
+  const int ky = t_kept_index(iy, g.ylo, g.yhi);
   const int t = threadIdx.x;
   const int c = t % CB;
   const int tid = t / CB;
@@ -365,7 +381,6 @@ __global__ void __launch_bounds__(kVCB*(NZ / P)) k_taz(TracerGeom g, TracerAdjAr
   const cplx<R>* tw = team_tw_load<NZ, P, R>(reinterpret_cast<cplx<R>*>(tsm) + CB * SL, a.twz);
   const int ix = tx * CB + c;
   const bool colkept = ix < g.KX;
-  const int iy = t_full_index(ky, g.ylo, g.yhi);
   const int64_t fbase = ((int64_t)tx * g.KY + ky) * NZ * CB + c;
   const int64_t sbase = ((int64_t)tx * g.KY + ky) * g.KZ * CB + c;
 
@@ -630,6 +645,9 @@ static size_t t_lds(int teams) {
           tw_slots<N, vpts_col(N)>()) * sizeof(cplx<R>);
 }
 
+// workgroups for `groups` groups of four XCD-sharing workgroups (xcd_split)
+static int t_grid4(int groups) { return (groups + kXCD - 1) / kXCD * kXCD * 4; }
+
 template <typename K>
 static int t_attr(K kern, size_t bytes) {
   FF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -654,7 +672,7 @@ static int launch_tz(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
   const double bytes = (STAGE > 0 ? 3 : 0) * nk * g.nz * cs + (a.do_inverse ? nk * g.nz * cs : 0) +
                        nk * g.KZ * (STAGE == 0 ? cs : (STAGE < 4 ? 3 * cs + 8 : 6 * cs + 8));
   prof_begin(names[STAGE], bytes, st);
-  hipLaunchKernelGGL(kern, dim3(g.TX * g.KY), dim3(kVCB * (N / P)), sh, st, g, a);
+  hipLaunchKernelGGL(kern, dim3(t_grid4(g.TX * (g.ny / 4))), dim3(kVCB * (N / P)), sh, st, g, a);
   prof_end(st);
   FF_LAUNCHED("k_tz");
   return FFHIP_OK;
@@ -674,7 +692,7 @@ static int launch_ty(const TracerGeom& g, const TracerYArgs& a, const char* name
   const double nk = (double)g.KX * g.nz;
   const double bytes = NF * nk * ((a.src_kept ? g.KY : g.ny) + (a.dst_kept ? g.KY : g.ny)) * cs;
   prof_begin(name, bytes, st);
-  hipLaunchKernelGGL(kern, dim3(g.TX * g.nz), dim3(kVCB * (N / P)), sh, st, g, a);
+  hipLaunchKernelGGL(kern, dim3(t_grid4(g.TX * (g.nz / 4))), dim3(kVCB * (N / P)), sh, st, g, a);
   prof_end(st);
   FF_LAUNCHED("k_ty");
   return FFHIP_OK;
@@ -769,7 +787,7 @@ static int launch_taz(const TracerGeom& g, const TracerAdjArgs& a, hipStream_t s
   const double nk = (double)g.KX * g.KY;
   const double bytes = (STAGE > 0 ? nk * g.nz * cs : 0) + 3 * nk * g.nz * cs + nk * g.KZ * (4 * cs + 8);
   prof_begin(names[STAGE], bytes, st);
-  hipLaunchKernelGGL(kern, dim3(g.TX * g.KY), dim3(kVCB * (N / P)), sh, st, g, a);
+  hipLaunchKernelGGL(kern, dim3(t_grid4(g.TX * (g.ny / 4))), dim3(kVCB * (N / P)), sh, st, g, a);
   prof_end(st);
   FF_LAUNCHED("k_taz");
   return FFHIP_OK;

@@ -44,6 +44,15 @@ void prof_end_impl(hipStream_t st) {
   if (!g_prof.empty()) (void)hipEventRecord(g_prof.back().b, st);
 }
 
+bool prof_pair(const char* name, double bytes, hipEvent_t* a, hipEvent_t* b) {
+  ProfRec r{name, bytes, prof_event(), prof_event()};
+  if (!r.a || !r.b) return false;
+  g_prof.push_back(r);
+  *a = r.a;
+  *b = r.b;
+  return true;
+}
+
 static thread_local std::string g_err;
 
 void clear_error() { g_err.clear(); }

@@ -671,9 +671,7 @@ static int launch_tz(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
   const double nk = (double)g.KX * g.KY;
   const double bytes = (STAGE > 0 ? 3 : 0) * nk * g.nz * cs + (a.do_inverse ? nk * g.nz * cs : 0) +
                        nk * g.KZ * (STAGE == 0 ? cs : (STAGE < 4 ? 3 * cs + 8 : 6 * cs + 8));
-  prof_begin(names[STAGE], bytes, st);
-  hipLaunchKernelGGL(kern, dim3(t_grid4(g.TX * (g.ny / 4))), dim3(kVCB * (N / P)), sh, st, g, a);
-  prof_end(st);
+  prof_launch(names[STAGE], bytes, kern, dim3(t_grid4(g.TX * (g.ny / 4))), dim3(kVCB * (N / P)), sh, st, g, a);
   FF_LAUNCHED("k_tz");
   return FFHIP_OK;
 }
@@ -691,9 +689,7 @@ static int launch_ty(const TracerGeom& g, const TracerYArgs& a, const char* name
   const double cs = sizeof(cplx<R>);
   const double nk = (double)g.KX * g.nz;
   const double bytes = NF * nk * ((a.src_kept ? g.KY : g.ny) + (a.dst_kept ? g.KY : g.ny)) * cs;
-  prof_begin(name, bytes, st);
-  hipLaunchKernelGGL(kern, dim3(t_grid4(g.TX * (g.nz / 4))), dim3(kVCB * (N / P)), sh, st, g, a);
-  prof_end(st);
+  prof_launch(name, bytes, kern, dim3(t_grid4(g.TX * (g.nz / 4))), dim3(kVCB * (N / P)), sh, st, g, a);
   FF_LAUNCHED("k_ty");
   return FFHIP_OK;
 }
@@ -702,15 +698,15 @@ template <typename R, int N>
 static int launch_tx(const TracerGeom& g, const TracerArgs& a, hipStream_t st) {
   constexpr int P = vpts_row(N);
   auto kern = k_tx<R, N, P>;
-  const size_t sh = ((size_t)4 * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), 4) + tw_slots<N, P>()) * sizeof(cplx<R>);
+  const size_t sh =
+      ((size_t)4 * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), 4) + tw_slots<N, P>()) * sizeof(cplx<R>);
   static bool set = false;
   if (!set) {
     FF_TRY(t_attr(kern, sh));
     set = true;
   }
-  prof_begin("tracer_x", 4.0 * g.KX * (double)g.ny * g.nz * sizeof(cplx<R>), st);
-  hipLaunchKernelGGL(kern, dim3(g.nz * (g.ny / 4)), dim3(4 * (N / P)), sh, st, g, a);
-  prof_end(st);
+  prof_launch("tracer_x", 4.0 * g.KX * (double)g.ny * g.nz * sizeof(cplx<R>), kern,
+              dim3(g.nz * (g.ny / 4)), dim3(4 * (N / P)), sh, st, g, a);
   FF_LAUNCHED("k_tx");
   return FFHIP_OK;
 }
@@ -786,9 +782,7 @@ static int launch_taz(const TracerGeom& g, const TracerAdjArgs& a, hipStream_t s
   const double cs = sizeof(cplx<R>);
   const double nk = (double)g.KX * g.KY;
   const double bytes = (STAGE > 0 ? nk * g.nz * cs : 0) + 3 * nk * g.nz * cs + nk * g.KZ * (4 * cs + 8);
-  prof_begin(names[STAGE], bytes, st);
-  hipLaunchKernelGGL(kern, dim3(t_grid4(g.TX * (g.ny / 4))), dim3(kVCB * (N / P)), sh, st, g, a);
-  prof_end(st);
+  prof_launch(names[STAGE], bytes, kern, dim3(t_grid4(g.TX * (g.ny / 4))), dim3(kVCB * (N / P)), sh, st, g, a);
   FF_LAUNCHED("k_taz");
   return FFHIP_OK;
 }
@@ -818,16 +812,16 @@ template <typename R, int N>
 static int launch_tax(const TracerGeom& g, TracerState* s, hipStream_t st) {
   constexpr int P = vpts_row(N);
   auto kern = k_tax<R, N, P>;
-  const size_t sh = ((size_t)4 * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), 4) + tw_slots<N, P>()) * sizeof(cplx<R>);
+  const size_t sh =
+      ((size_t)4 * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), 4) + tw_slots<N, P>()) * sizeof(cplx<R>);
   static bool set = false;
   if (!set) {
     FF_TRY(t_attr(kern, sh));
     set = true;
   }
-  prof_begin("tracer_adj_x", 4.0 * g.KX * (double)g.ny * g.nz * sizeof(cplx<R>), st);
-  hipLaunchKernelGGL(kern, dim3(g.nz * (g.ny / 4)), dim3(4 * (N / P)), sh, st, g, (const void*)s->Bf[0],
-                     (const void*)s->Bf[1], (const void*)s->Bf[2], s->A2, s->tab, (const void*)s->twx);
-  prof_end(st);
+  prof_launch("tracer_adj_x", 4.0 * g.KX * (double)g.ny * g.nz * sizeof(cplx<R>), kern,
+              dim3(g.nz * (g.ny / 4)), dim3(4 * (N / P)), sh, st, g, (const void*)s->Bf[0],
+              (const void*)s->Bf[1], (const void*)s->Bf[2], s->A2, s->tab, (const void*)s->twx);
   FF_LAUNCHED("k_tax");
   return FFHIP_OK;
 }

@@ -311,10 +311,8 @@ static int launch_acol(const VortGeom& g, const VortAdjColArgs& a, hipStream_t s
                        nc * (STAGE == 0 ? 4 * cs + 4 * ds : STAGE == 1 ? 5 * cs + 2 * ds
                                           : STAGE == 2 ? 3 * cs + 2 * ds : STAGE == 3 ? 5 * cs + 2 * ds
                                           : (a.cont ? 4 * cs + 4 * ds : 2 * cs));
-  prof_begin(names[STAGE], bytes, st);
   const int groups = (g.TX + kXCD - 1) / kXCD * kXCD;
-  hipLaunchKernelGGL(kern, dim3(groups * kCSUB), dim3(kCBK * (N / P)), sh, st, g, a);
-  prof_end(st);
+  prof_launch(names[STAGE], bytes, kern, dim3(groups * kCSUB), dim3(kCBK * (N / P)), sh, st, g, a);
   FF_LAUNCHED("k_vacol");
   return FFHIP_OK;
 }
@@ -332,10 +330,8 @@ static int launch_arow(const VortGeom& g, const VortAdjRowArgs& a, hipStream_t s
     attr_set = true;
   }
   const double bytes = 5.0 * g.KX * g.ny * sizeof(cplx<R>) + (double)vfield_bytes<R>(g);
-  prof_begin("vort_adj_row", bytes, st);
   const int groups = (g.ny / 4 + kXCD - 1) / kXCD * kXCD;
-  hipLaunchKernelGGL(kern, dim3(groups * kRSUB), dim3(kRRK * (N / P)), sh, st, g, a);
-  prof_end(st);
+  prof_launch("vort_adj_row", bytes, kern, dim3(groups * kRSUB), dim3(kRRK * (N / P)), sh, st, g, a);
   FF_LAUNCHED("k_varow");
   return FFHIP_OK;
 }

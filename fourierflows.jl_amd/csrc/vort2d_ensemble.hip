@@ -199,9 +199,7 @@ static int ens_pass(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only
   if (which == 5) {
     auto k = k_ens_row<R, BD, PRULE>;
     if (attr_only) return set_lds(k, e->lds_row);
-    prof_begin(names[5], e->bytes[5], st);
-    hipLaunchKernelGGL(k, dim3(e->nrow), dim3(BD), e->lds_row, st, mem, items + e->ncol);
-    prof_end(st);
+    prof_launch(names[5], e->bytes[5], k, dim3(e->nrow), dim3(BD), e->lds_row, st, mem, items + e->ncol);
     FF_LAUNCHED("k_ens_row");
     return FFHIP_OK;
   }
@@ -214,9 +212,7 @@ static int ens_pass(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only
     default: k = k_ens_col<R, BD, PRULE, 4>; break;
   }
   if (attr_only) return set_lds(k, e->lds_col);
-  prof_begin(names[which], e->bytes[which], st);
-  hipLaunchKernelGGL(k, dim3(e->ncol), dim3(BD), e->lds_col, st, mem, items);
-  prof_end(st);
+  prof_launch(names[which], e->bytes[which], k, dim3(e->ncol), dim3(BD), e->lds_col, st, mem, items);
   FF_LAUNCHED("k_ens_col");
   return FFHIP_OK;
 }

@@ -133,10 +133,9 @@ static int launch_col(const VortGeom& g, const VortColArgs& a, hipStream_t st) {
   }
   static const char* names[5] = {"vort_col_stage0", "vort_col_stage1", "vort_col_stage2", "vort_col_stage3",
                                  "vort_col_stage4"};
-  prof_begin(names[STAGE], col_bytes<R>(g, STAGE, a.do_inverse), st);
   const int groups = (g.TX + kXCD - 1) / kXCD * kXCD;
-  hipLaunchKernelGGL(kern, dim3(groups * kCSUB), dim3(kCBK * (N / P)), sh, st, g, a);
-  prof_end(st);
+  prof_launch(names[STAGE], col_bytes<R>(g, STAGE, a.do_inverse), kern,
+              dim3(groups * kCSUB), dim3(kCBK * (N / P)), sh, st, g, a);
   FF_LAUNCHED("k_vcol");
   return FFHIP_OK;
 }
@@ -171,10 +170,9 @@ static int launch_col3(const VortGeom& g, const VortColArgs& a, hipStream_t st) 
   }
   static const char* names[5] = {"vort_col_stage0", "vort_col_stage1", "vort_col_stage2", "vort_col_stage3",
                                  "vort_col_stage4"};
-  prof_begin(names[STAGE], col_bytes<R>(g, STAGE, a.do_inverse), st);
   const int groups = (g.TX + kXCD - 1) / kXCD * kXCD;
-  hipLaunchKernelGGL(kern, dim3(groups * kVCB), dim3(3 * (N / P)), sh, st, g, a);
-  prof_end(st);
+  prof_launch(names[STAGE], col_bytes<R>(g, STAGE, a.do_inverse), kern,
+              dim3(groups * kVCB), dim3(3 * (N / P)), sh, st, g, a);
   FF_LAUNCHED("k_vcol3");
   return FFHIP_OK;
 }
@@ -190,10 +188,8 @@ static int launch_row3(const VortGeom& g, const VortRowArgs& a, hipStream_t st) 
                                (int)sh));
     attr_set = true;
   }
-  prof_begin("vort_row", row_bytes<R>(g), st);
   const int groups = (g.ny / 4 + kXCD - 1) / kXCD * kXCD;
-  hipLaunchKernelGGL(kern, dim3(groups * 2), dim3(3 * (N / P)), sh, st, g, a);
-  prof_end(st);
+  prof_launch("vort_row", row_bytes<R>(g), kern, dim3(groups * 2), dim3(3 * (N / P)), sh, st, g, a);
   FF_LAUNCHED("k_vrow3");
   return FFHIP_OK;
 }
@@ -209,10 +205,8 @@ static int launch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
                                (int)sh));
     attr_set = true;
   }
-  prof_begin("vort_row", row_bytes<R>(g), st);
   const int groups = (g.ny / 4 + kXCD - 1) / kXCD * kXCD;
-  hipLaunchKernelGGL(kern, dim3(groups * kRSUB), dim3(kRRK * (N / P)), sh, st, g, a);
-  prof_end(st);
+  prof_launch("vort_row", row_bytes<R>(g), kern, dim3(groups * kRSUB), dim3(kRRK * (N / P)), sh, st, g, a);
   FF_LAUNCHED("k_vrow");
   return FFHIP_OK;
 }

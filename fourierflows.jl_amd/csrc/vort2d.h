@@ -49,6 +49,19 @@ __device__ __forceinline__ void xcd_split(int b, int nsub, int* group, int* sub)
   *group = (slot / nsub) * kXCD + x;
 }
 
+// Element idx of an array addressed by a 32-bit byte offset from its base
+// (valid for arrays under 4 GiB): the load or store takes the uniform base in
+// scalar registers and one 32-bit offset register, with no 64-bit address
+// arithmetic per element.
+template <typename T>
+__device__ __forceinline__ T& at32(T* base, uint32_t idx) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ const T& at32(const T* base, uint32_t idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+
 // points per thread of the y (column) and x (row) transforms
 constexpr int vpts_col(int n) { return n >= 256 ? 16 : 8; }
 constexpr int vpts_row(int n) { return n >= 256 ? 16 : 8; }

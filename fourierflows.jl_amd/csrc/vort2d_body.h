@@ -90,8 +90,8 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
   const int ix = tx * CB + c;
   const bool colkept = live && ix < g.KX;
   const R kx = (R)(g.dk * (double)ix);
-  const int64_t ibase = (int64_t)tx * NY * CB + c;
-  const int64_t sbase = (int64_t)tx * g.KY * CB + c;
+  const uint32_t ibase = (uint32_t)tx * NY * CB + c;  // 32-bit element offsets (at32)
+  const uint32_t sbase = (uint32_t)tx * g.KY * CB + c;
   // the five transforms share one set of twiddles: FP32 keeps the first
   // pass's factors in registers (all passes would cost the second wavefront)
   constexpr bool treg = sizeof(R) == 4 && FF_COL_TWREG;
@@ -116,10 +116,10 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
     const cplx<R>* O0 = reinterpret_cast<const cplx<R>*>(a.O0);
     const cplx<R>* O1 = reinterpret_cast<const cplx<R>*>(a.O1);
 #pragma unroll
-    for (int p = 0; p < P; ++p) nn[p] = live ? O0[ibase + (int64_t)(tid + p * T) * CB] : mkc<R>(0, 0);
+    for (int p = 0; p < P; ++p) nn[p] = live ? at32(O0, ibase + (uint32_t)(tid + p * T) * CB) : mkc<R>(0, 0);
     fft(nn, Fwd{});
 #pragma unroll
-    for (int p = 0; p < P; ++p) vb[p] = live ? O1[ibase + (int64_t)(tid + p * T) * CB] : mkc<R>(0, 0);
+    for (int p = 0; p < P; ++p) vb[p] = live ? at32(O1, ibase + (uint32_t)(tid + p * T) * CB) : mkc<R>(0, 0);
     fft(vb, Fwd{});
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -138,7 +138,7 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
     const bool kept = colkept && (iy < g.ylo || iy > g.yhi);
     nw[p] = mkc<R>(0, 0);
     if (!kept) continue;
-    const int64_t si = sbase + (int64_t)v_ky(iy, g) * CB;
+    const int64_t si = (int64_t)sbase + (int64_t)v_ky(iy, g) * CB;
     nw[p] = vstage_point<R, STAGE>(a, si, nn[p]);
   }
   if (!a.do_inverse) return;  // uniform over the block
@@ -158,7 +158,7 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
     cplx<R>* d = dst[f];
     if (live) {
 #pragma unroll
-      for (int p = 0; p < P; ++p) d[ibase + (int64_t)(tid + p * T) * CB] = v[p];
+      for (int p = 0; p < P; ++p) at32(d, ibase + (uint32_t)(tid + p * T) * CB) = v[p];
     }
   }
 }
@@ -247,7 +247,9 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
   constexpr int T = NX / P;
   constexpr int CB = kVCB;
   const int64_t ny = g.ny;
-  auto iaddr = [&](int k) -> int64_t { return ((int64_t)(k / CB) * ny + y) * CB + (k % CB); };
+  // 32-bit element offsets from the array bases (the blocked arrays hold
+  // fewer than 2^32 elements): saddr loads, no 64-bit address arithmetic
+  auto iaddr = [&](int k) -> uint32_t { return ((uint32_t)(k / CB) * (uint32_t)ny + (uint32_t)y) * CB + (k % CB); };
   const cplx<R>* Zh = reinterpret_cast<const cplx<R>*>(a.I0);
   const cplx<R>* Uh = reinterpret_cast<const cplx<R>*>(a.I1);
   const cplx<R>* Vh = reinterpret_cast<const cplx<R>*>(a.I2);
@@ -272,14 +274,14 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
     const int k = tid + p * T;
     cplx<R> s = mkc<R>(0, 0);
     if (k == 0) {
-      const cplx<R> U = Uh[iaddr(0)], V = Vh[iaddr(0)];
+      const cplx<R> U = at32(Uh, iaddr(0)), V = at32(Vh, iaddr(0));
       s = mkc<R>(U.x, V.x);  // c2r ignores Im at kx = 0
     } else if (k < KX) {
-      const cplx<R> U = Uh[iaddr(k)], V = Vh[iaddr(k)];
+      const cplx<R> U = at32(Uh, iaddr(k)), V = at32(Vh, iaddr(k));
       s = mkc<R>(U.x - V.y, U.y + V.x);
     } else if (k > NX - KX) {
       const int k2 = NX - k;
-      const cplx<R> U = Uh[iaddr(k2)], V = Vh[iaddr(k2)];
+      const cplx<R> U = at32(Uh, iaddr(k2)), V = at32(Vh, iaddr(k2));
       s = mkc<R>(U.x + V.y, V.x - U.y);  // conj(U) + i conj(V)
     }
     z[p] = s;
@@ -295,11 +297,11 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
     const int k = tid + p * T;
     cplx<R> s = mkc<R>(0, 0);
     if (k == 0) {
-      s = mkc<R>(Zh[iaddr(0)].x, 0);
+      s = mkc<R>(at32(Zh, iaddr(0)).x, 0);
     } else if (k < KX) {
-      s = Zh[iaddr(k)];
+      s = at32(Zh, iaddr(k));
     } else if (k > NX - KX) {
-      s = cconj<R>(Zh[iaddr(NX - k)]);
+      s = cconj<R>(at32(Zh, iaddr(NX - k)));
     }
     w[p] = s;
   }
@@ -342,8 +344,8 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
       Pk = mkc<R>(R(0.5) * (wk.x + wc.x), R(0.5) * (wk.y + wc.y));
       Qk = mkc<R>(R(0.5) * (wk.y - wc.y), -R(0.5) * (wk.x - wc.x));  // (wk - wc) / (2i)
     }
-    O0[iaddr(k)] = Pk;
-    O1[iaddr(k)] = Qk;
+    at32(O0, iaddr(k)) = Pk;
+    at32(O1, iaddr(k)) = Qk;
   }
 }
 

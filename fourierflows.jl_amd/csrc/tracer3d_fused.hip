@@ -26,6 +26,10 @@
 
 namespace ffhip {
 
+#ifndef FF_TAX_TWREG
+#define FF_TAX_TWREG 1  // adjoint x pass: twiddles in registers across its three transforms
+#endif
+
 struct TracerGeom {
   int nx, ny, nz, nkr;
   int KX, KXp, TX;
@@ -264,17 +268,13 @@ __global__ void __launch_bounds__(4 * (NX / P)) k_tx(TracerGeom g, TracerArgs a)
   // c (real) as one complex inverse of its Hermitian-extended half spectrum
   cplx<R> w[P];
 #pragma unroll
-  for (int p = 0; p < P; ++p) {
+  for (int p = 0; p < P; ++p) {  // branch-free: all loads issue before the first wait
     const int k = tid + p * T;
-    cplx<R> s = mkc<R>(0, 0);
-    if (k == 0) {
-      s = mkc<R>(A2[xaddr(0)].x, 0);  // c2r ignores Im at kx = 0
-    } else if (k < KX) {
-      s = A2[xaddr(k)];
-    } else if (k > NX - KX) {
-      s = cconj<R>(A2[xaddr(NX - k)]);
-    }
-    w[p] = s;
+    const bool lo = k < KX, hi = k > NX - KX;  // direct / mirrored (conjugate) source
+    const cplx<R> c = A2[xaddr(lo ? k : (hi ? NX - k : 0))];
+    const R sx = (lo || hi) ? c.x : R(0);
+    const R sy = lo ? (k == 0 ? R(0) : c.y) : (hi ? -c.y : R(0));  // c2r ignores Im at kx = 0
+    w[p] = mkc<R>(sx, sy);
   }
   team_fft<NX, P, 1, R>(w, lds, tid, tw);
 
@@ -474,26 +474,24 @@ __global__ void __launch_bounds__(4 * (NX / P)) k_tax(TracerGeom g, const void* 
   const cplx<R>* Rh = reinterpret_cast<const cplx<R>*>(s2);
   cplx<R> z[P], w[P];
 #pragma unroll
-  for (int p = 0; p < P; ++p) {
+  for (int p = 0; p < P; ++p) {  // branch-free: all loads issue before the first wait
     const int k = tid + p * T;
-    cplx<R> sv = mkc<R>(0, 0), sr = mkc<R>(0, 0);
-    if (k == 0) {
-      sv = mkc<R>(Ph[xaddr(0)].x, Qh[xaddr(0)].x);  // c2r ignores Im at kx = 0
-      sr = mkc<R>(Rh[xaddr(0)].x, 0);
-    } else if (k < KX) {
-      const cplx<R> A = Ph[xaddr(k)], B = Qh[xaddr(k)];
-      sv = mkc<R>(A.x - B.y, A.y + B.x);
-      sr = Rh[xaddr(k)];
-    } else if (k > NX - KX) {
-      const cplx<R> A = Ph[xaddr(NX - k)], B = Qh[xaddr(NX - k)];
-      sv = mkc<R>(A.x + B.y, B.x - A.y);
-      sr = cconj<R>(Rh[xaddr(NX - k)]);
-    }
-    z[p] = sv;
-    w[p] = sr;
+    const bool lo = k < KX, hi = k > NX - KX;  // direct / mirrored (conjugate) source
+    const int64_t q = xaddr(lo ? k : (hi ? NX - k : 0));
+    const cplx<R> A = Ph[q], B = Qh[q], C = Rh[q];
+    const R ay = k == 0 ? R(0) : A.y, by = k == 0 ? R(0) : B.y, cy = k == 0 ? R(0) : C.y;  // Im ignored at kx = 0
+    z[p] = mkc<R>(lo ? A.x - by : (hi ? A.x + B.y : R(0)), lo ? ay + B.x : (hi ? B.x - A.y : R(0)));
+    w[p] = mkc<R>((lo || hi) ? C.x : R(0), lo ? cy : (hi ? -C.y : R(0)));
   }
+#if FF_TAX_TWREG
+  TwRegs<NX, P, R> twr;  // the three transforms share the thread's twiddles
+  twr_fill(twr, tw, tid);
+  team_fft<NX, P, 1, R>(z, lds, tid, tw, twr);  // p + i q
+  team_fft<NX, P, 1, R>(w, lds, tid, tw, twr);  // r
+#else
   team_fft<NX, P, 1, R>(z, lds, tid, tw);  // p + i q
   team_fft<NX, P, 1, R>(w, lds, tid, tw);  // r
+#endif
   const double* sx = tab;
   const double* cx = tab + g.nx;
   const double* sy = tab + 2 * g.nx;
@@ -509,7 +507,11 @@ __global__ void __launch_bounds__(4 * (NX / P)) k_tax(TracerGeom g, const void* 
     const R ww = (R)(csy + g.B * cx[ix]);
     z[p] = mkc<R>(u * z[p].x + v * z[p].y + ww * w[p].x, 0);  // u p + v q + w r
   }
+#if FF_TAX_TWREG
+  team_fft<NX, P, -1, R>(z, lds, tid, tw, twr);
+#else
   team_fft<NX, P, -1, R>(z, lds, tid, tw);
+#endif
   cplx<R>* D = reinterpret_cast<cplx<R>*>(dst);
 #pragma unroll
   for (int p = 0; p < P; ++p) {

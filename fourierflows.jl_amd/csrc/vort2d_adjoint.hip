@@ -227,19 +227,13 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_varow(VortGeom g, VortAdjRowA
   // p + i q = c2r of the two half spectra, packed (unnormalised inverses are mul!^T exactly)
   cplx<R> z[P];
 #pragma unroll
-  for (int p = 0; p < P; ++p) {
+  for (int p = 0; p < P; ++p) {  // branch-free: all loads issue before the first wait
     const int k = tid + p * T;
-    cplx<R> s = mkc<R>(0, 0);
-    if (k == 0) {
-      s = mkc<R>(Ph[iaddr(0)].x, Qh[iaddr(0)].x);
-    } else if (k < KX) {
-      const cplx<R> A = Ph[iaddr(k)], B = Qh[iaddr(k)];
-      s = mkc<R>(A.x - B.y, A.y + B.x);
-    } else if (k > NX - KX) {
-      const cplx<R> A = Ph[iaddr(NX - k)], B = Qh[iaddr(NX - k)];
-      s = mkc<R>(A.x + B.y, B.x - A.y);
-    }
-    z[p] = s;
+    const bool lo = k < KX, hi = k > NX - KX;  // direct / mirrored (conjugate) source
+    const int kk = lo ? k : (hi ? NX - k : 0);
+    const cplx<R> A = at32(Ph, iaddr(kk)), B = at32(Qh, iaddr(kk));
+    const R ay = k == 0 ? R(0) : A.y, by = k == 0 ? R(0) : B.y;  // Im ignored at kx = 0
+    z[p] = mkc<R>(lo ? A.x - by : (hi ? A.x + B.y : R(0)), lo ? ay + B.x : (hi ? B.x - A.y : R(0)));
   }
   team_fft<NX, P, 1, R>(z, lds, tid, tw);
 

@@ -267,24 +267,21 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
   using Inv = std::integral_constant<int, 1>;
   using Fwd = std::integral_constant<int, -1>;
 
-  // u + i v: Hermitian-extended half spectra packed into one complex inverse
+  // u + i v: Hermitian-extended half spectra packed into one complex inverse.
+  // Branch-free: every lane loads (the zero band reads column 0 and discards
+  // it), so the compiler issues all loads before the first wait instead of
+  // one round trip per point.
   cplx<R> z[P];
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     const int k = tid + p * T;
-    cplx<R> s = mkc<R>(0, 0);
-    if (k == 0) {
-      const cplx<R> U = at32(Uh, iaddr(0)), V = at32(Vh, iaddr(0));
-      s = mkc<R>(U.x, V.x);  // c2r ignores Im at kx = 0
-    } else if (k < KX) {
-      const cplx<R> U = at32(Uh, iaddr(k)), V = at32(Vh, iaddr(k));
-      s = mkc<R>(U.x - V.y, U.y + V.x);
-    } else if (k > NX - KX) {
-      const int k2 = NX - k;
-      const cplx<R> U = at32(Uh, iaddr(k2)), V = at32(Vh, iaddr(k2));
-      s = mkc<R>(U.x + V.y, V.x - U.y);  // conj(U) + i conj(V)
-    }
-    z[p] = s;
+    const bool lo = k < KX, hi = k > NX - KX;  // direct / mirrored (conjugate) source
+    const int kk = lo ? k : (hi ? NX - k : 0);
+    const cplx<R> U = at32(Uh, iaddr(kk)), V = at32(Vh, iaddr(kk));
+    const R uy = k == 0 ? R(0) : U.y, vy = k == 0 ? R(0) : V.y;  // c2r ignores Im at kx = 0
+    const R sx = lo ? U.x - vy : (hi ? U.x + V.y : R(0));
+    const R sy = lo ? uy + V.x : (hi ? V.x - U.y : R(0));  // conj(U) + i conj(V) when mirrored
+    z[p] = mkc<R>(sx, sy);
   }
   // FP64: issue the zeta^ loads before the u + i v transform (measured faster);
   // FP32: transform first (with the twiddles in registers, measured 3% faster)
@@ -295,15 +292,12 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     const int k = tid + p * T;
-    cplx<R> s = mkc<R>(0, 0);
-    if (k == 0) {
-      s = mkc<R>(at32(Zh, iaddr(0)).x, 0);
-    } else if (k < KX) {
-      s = at32(Zh, iaddr(k));
-    } else if (k > NX - KX) {
-      s = cconj<R>(at32(Zh, iaddr(NX - k)));
-    }
-    w[p] = s;
+    const bool lo = k < KX, hi = k > NX - KX;
+    const int kk = lo ? k : (hi ? NX - k : 0);
+    const cplx<R> Z = at32(Zh, iaddr(kk));
+    const R sx = (lo || hi) ? Z.x : R(0);
+    const R sy = lo ? (k == 0 ? R(0) : Z.y) : (hi ? -Z.y : R(0));
+    w[p] = mkc<R>(sx, sy);
   }
   if constexpr (prefetch) fft(z, Inv{});
   fft(w, Inv{});
@@ -381,18 +375,12 @@ __device__ __forceinline__ void vrow3_body(const VortGeom& g, const VortRowArgs&
   for (int p = 0; p < P; ++p) {
     const int k = tid + p * T;
     const bool lo = k < KX, hi = k > NX - KX;
-    const int kk = lo ? k : NX - k;
-    cplx<R> s = mkc<R>(0, 0);
-    if (lo || hi) {
-      cplx<R> A = SA[iaddr(kk, ya)], B = SB[iaddr(kk, yb)];
-      if (k == 0) A.y = B.y = R(0);  // c2r ignores Im at kx = 0
-      if (hi) {
-        A.y = -A.y;  // conj
-        B.y = -B.y;
-      }
-      s = mkc<R>(A.x - B.y, A.y + B.x);
-    }
-    v[p] = s;
+    const int kk = lo ? k : (hi ? NX - k : 0);  // branch-free: the zero band reads column 0
+    const cplx<R> A = at32(SA, iaddr(kk, ya)), B = at32(SB, iaddr(kk, yb));
+    // c2r ignores Im at kx = 0; mirrored modes are conjugated
+    const R ay = k == 0 ? R(0) : (hi && !lo ? -A.y : A.y), by = k == 0 ? R(0) : (hi && !lo ? -B.y : B.y);
+    const bool any = lo || hi;
+    v[p] = mkc<R>(any ? A.x - by : R(0), any ? ay + B.x : R(0));
   }
   team_fft<NX, P, 1, R>(v, mine, tid, tw);
   if (team == 2) {

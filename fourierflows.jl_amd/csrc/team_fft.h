@@ -181,13 +181,39 @@ __device__ __forceinline__ void twr_fill(TwRegs<N, P, R, NREG>& t, const cplx<R>
   twr_fill<N, P, 1, R, NREG>(t, ltw, tid);
 }
 
-template <int N, int P, int RK, int NS, int TWO, int TRO, bool REG, bool FIRST, bool LAST, int DIR, typename R>
-__device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__ lds, int tid,
-                                          const cplx<R>* __restrict__ tw, const cplx<R>* twr) {
+// The same registers filled straight from the N roots of unity in global
+// memory (for kernels that keep no LDS table).
+template <int N, int P, int S, typename R, int NREG>
+__device__ __forceinline__ void twr_fill_roots(TwRegs<N, P, R, NREG>& t, const cplx<R>* roots, int tid) {
+  using TS = TeamSched<N, P>;
+  if constexpr (S < TS::NPASS && S <= NREG) {
+    constexpr int RK = S < TS::NFULL ? P : TS::REM;
+    constexpr int NS = cpow(P, S);
+    constexpr int NB = P / RK;
+    constexpr int T = N / P;
+    constexpr int STEP = N / (NS * RK);
+    constexpr int RO = twr_off<N, P, S>();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int k = (tid + b * T) & (NS - 1);
+#pragma unroll
+      for (int r = 1; r < RK; ++r) t.w[RO + b * (RK - 1) + r - 1] = roots[r * k * STEP];
+    }
+    twr_fill_roots<N, P, S + 1, R, NREG>(t, roots, tid);
+  }
+}
+
+// One pass in two halves: tp_load gathers a thread's butterfly inputs (from
+// registers on the first pass, else from LDS), tp_exec applies twiddles and
+// butterflies and writes the outputs (to registers on the last pass, else to
+// LDS).  team_pass puts the barriers between them; team_pass2 runs two
+// independent transforms through one set of barriers.
+template <int N, int P, int RK, bool FIRST, typename R>
+__device__ __forceinline__ void tp_load(const cplx<R> (&v)[P], cplx<R> (&x)[P], const cplx<R>* __restrict__ lds,
+                                        int tid) {
   constexpr int T = N / P;
   constexpr int NB = P / RK;  // butterflies per thread
   constexpr int M = N / RK;
-  cplx<R> x[P];
   if constexpr (FIRST) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
@@ -198,8 +224,14 @@ __device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__
     for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int r = 0; r < RK; ++r) x[b * RK + r] = lds[lpad(tid + b * T + r * M)];
-    __syncthreads();
   }
+}
+
+template <int N, int P, int RK, int NS, int TWO, int TRO, bool REG, bool LAST, int DIR, typename R>
+__device__ __forceinline__ void tp_exec(cplx<R> (&v)[P], cplx<R> (&x)[P], cplx<R>* __restrict__ lds, int tid,
+                                        const cplx<R>* __restrict__ tw, const cplx<R>* twr) {
+  constexpr int T = N / P;
+  constexpr int NB = P / RK;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = tid + b * T;
@@ -222,6 +254,28 @@ __device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__
       for (int r = 0; r < RK; ++r) lds[lpad(o + r * NS)] = y[r];
     }
   }
+}
+
+template <int N, int P, int RK, int NS, int TWO, int TRO, bool REG, bool FIRST, bool LAST, int DIR, typename R>
+__device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__ lds, int tid,
+                                          const cplx<R>* __restrict__ tw, const cplx<R>* twr) {
+  cplx<R> x[P];
+  tp_load<N, P, RK, FIRST, R>(v, x, lds, tid);
+  if constexpr (!FIRST) __syncthreads();
+  tp_exec<N, P, RK, NS, TWO, TRO, REG, LAST, DIR, R>(v, x, lds, tid, tw, twr);
+  if constexpr (!LAST) __syncthreads();
+}
+
+template <int N, int P, int RK, int NS, int TWO, int TRO, bool REG, bool FIRST, bool LAST, int DIR, typename R>
+__device__ __forceinline__ void team_pass2(cplx<R> (&v1)[P], cplx<R> (&v2)[P], cplx<R>* __restrict__ lds1,
+                                           cplx<R>* __restrict__ lds2, int tid, const cplx<R>* __restrict__ tw,
+                                           const cplx<R>* twr) {
+  cplx<R> x1[P], x2[P];
+  tp_load<N, P, RK, FIRST, R>(v1, x1, lds1, tid);
+  tp_load<N, P, RK, FIRST, R>(v2, x2, lds2, tid);
+  if constexpr (!FIRST) __syncthreads();
+  tp_exec<N, P, RK, NS, TWO, TRO, REG, LAST, DIR, R>(v1, x1, lds1, tid, tw, twr);
+  tp_exec<N, P, RK, NS, TWO, TRO, REG, LAST, DIR, R>(v2, x2, lds2, tid, tw, twr);
   if constexpr (!LAST) __syncthreads();
 }
 
@@ -238,6 +292,19 @@ __device__ __forceinline__ void team_passes(cplx<R> (&v)[P], cplx<R>* lds, int t
   }
 }
 
+template <int N, int P, int S, int DIR, int NREG, typename R>
+__device__ __forceinline__ void team_passes2(cplx<R> (&v1)[P], cplx<R> (&v2)[P], cplx<R>* lds1, cplx<R>* lds2,
+                                             int tid, const cplx<R>* tw, const cplx<R>* twr) {
+  using TS = TeamSched<N, P>;
+  if constexpr (S < TS::NPASS) {
+    constexpr int RK = S < TS::NFULL ? P : TS::REM;
+    constexpr int NS = cpow(P, S);
+    team_pass2<N, P, RK, NS, tw_off<N, P, S>(), twr_off<N, P, S>(), (S <= NREG), S == 0, S == TS::NPASS - 1, DIR,
+               R>(v1, v2, lds1, lds2, tid, tw, twr);
+    team_passes2<N, P, S + 1, DIR, NREG, R>(v1, v2, lds1, lds2, tid, tw, twr);
+  }
+}
+
 // Full transform.  Requires NPASS >= 2 (N >= 2P) so that the first pass
 // writes LDS and the last pass reads it.  Ends with LDS free for reuse.
 // tw is the table team_tw_load filled.
@@ -245,6 +312,16 @@ template <int N, int P, int DIR, typename R>
 __device__ __forceinline__ void team_fft(cplx<R> (&v)[P], cplx<R>* lds, int tid, const cplx<R>* tw) {
   static_assert(TeamSched<N, P>::NPASS >= 2, "team_fft needs at least two passes");
   team_passes<N, P, 0, DIR, 0, R>(v, lds, tid, tw, nullptr);
+}
+
+// Two independent transforms of one direction side by side (separate LDS
+// buffers, shared barriers: half the barriers of two team_fft calls and two
+// streams of independent work between them), twiddles as below.
+template <int N, int P, int DIR, typename R, int NREG>
+__device__ __forceinline__ void team_fft2(cplx<R> (&v1)[P], cplx<R> (&v2)[P], cplx<R>* lds1, cplx<R>* lds2, int tid,
+                                          const cplx<R>* tw, const TwRegs<N, P, R, NREG>& t) {
+  static_assert(TeamSched<N, P>::NPASS >= 2, "team_fft needs at least two passes");
+  team_passes2<N, P, 0, DIR, NREG, R>(v1, v2, lds1, lds2, tid, tw, t.w);
 }
 
 // The same transform with the first t.NR passes' twiddles from registers

@@ -241,9 +241,14 @@ __device__ __forceinline__ void vcol3_body(const VortGeom& g, const VortColArgs&
 
 // One x-row y of the row pass, run by a team of NX/P threads (team index
 // tid) with its own LDS slice.
-template <typename R, int NX, int P>
+// PAIR (FP32 only): the two inverse transforms run side by side through
+// team_fft2 in lds and lds2, with every twiddle in registers filled from the
+// roots of unity tw (global memory); otherwise tw is the LDS table and lds2
+// is unused.
+template <typename R, int NX, int P, bool PAIR = false>
 __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& a, const int y, const int tid,
-                                          cplx<R>* lds, const cplx<R>* tw) {
+                                          cplx<R>* lds, const cplx<R>* tw, cplx<R>* lds2 = nullptr) {
+  static_assert(!PAIR || sizeof(R) == 4, "paired row transforms are FP32 only");
   constexpr int T = NX / P;
   constexpr int CB = kVCB;
   const int64_t ny = g.ny;
@@ -255,9 +260,12 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
   const cplx<R>* Vh = reinterpret_cast<const cplx<R>*>(a.I2);
   const int KX = g.KX;
   // the three transforms share one set of twiddles: FP32 keeps it in registers
-  constexpr bool treg = sizeof(R) == 4 && FF_ROW_TWREG;
+  constexpr bool treg = PAIR || (sizeof(R) == 4 && FF_ROW_TWREG);
   TwRegs<NX, P, R> twr;
-  if constexpr (treg) twr_fill(twr, tw, tid);
+  if constexpr (PAIR)
+    twr_fill_roots<NX, P, 1, R>(twr, tw, tid);
+  else if constexpr (treg)
+    twr_fill(twr, tw, tid);
   auto fft = [&](cplx<R>(&x)[P], auto dir) {
     if constexpr (treg)
       team_fft<NX, P, decltype(dir)::value, R>(x, lds, tid, tw, twr);
@@ -285,7 +293,7 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
   }
   // FP64: issue the zeta^ loads before the u + i v transform (measured faster);
   // FP32: transform first (with the twiddles in registers, measured 3% faster)
-  constexpr bool prefetch = sizeof(R) == 8 ? FF_ROW_PREFETCH_F64 : FF_ROW_PREFETCH_F32;
+  constexpr bool prefetch = PAIR || (sizeof(R) == 8 ? FF_ROW_PREFETCH_F64 : FF_ROW_PREFETCH_F32);
   if constexpr (!prefetch) fft(z, Inv{});
 
   cplx<R> w[P];
@@ -299,8 +307,12 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
     const R sy = lo ? (k == 0 ? R(0) : Z.y) : (hi ? -Z.y : R(0));
     w[p] = mkc<R>(sx, sy);
   }
-  if constexpr (prefetch) fft(z, Inv{});
-  fft(w, Inv{});
+  if constexpr (PAIR) {
+    team_fft2<NX, P, 1, R>(z, w, lds, lds2, tid, tw, twr);
+  } else {
+    if constexpr (prefetch) fft(z, Inv{});
+    fft(w, Inv{});
+  }
 
   // products u zeta + i v zeta with the 1/(nx ny)^2 normalisation
   const R s2 = (R)g.s2;

@@ -33,6 +33,14 @@
 namespace ffhip {
 
 
+#ifndef FF_ROW_PAIR
+#define FF_ROW_PAIR 1  // FP32 single-grid row pass: the two inverse transforms side by side
+#endif
+template <typename R>
+constexpr bool row_pair() {
+  return sizeof(R) == 4 && FF_ROW_PAIR;
+}
+
 #ifndef FF_ROW_WAVES
 #define FF_ROW_WAVES 1  // minimum FP32 waves per SIMD the single-team row pass is compiled for
 #endif
@@ -101,8 +109,13 @@ __global__ void __launch_bounds__(kRRK*(NX / P), sizeof(R) == 4 ? FF_ROW_WAVES :
   const int r = t % kRRK;
   constexpr int SL = team_slice(TeamSched<NX, P>::LDS, sizeof(cplx<R>), kRRK);
   cplx<R>* base = reinterpret_cast<cplx<R>*>(vsm);
-  const cplx<R>* tw = team_tw_load<NX, P, R>(base + kRRK * SL, a.tw);
-  vrow_body<R, NX, P>(g, a, grp * 4 + sub * kRRK + r, t / kRRK, base + r * SL, tw);
+  if constexpr (row_pair<R>()) {  // second buffers after the first, twiddles in registers
+    vrow_body<R, NX, P, true>(g, a, grp * 4 + sub * kRRK + r, t / kRRK, base + r * SL,
+                              reinterpret_cast<const cplx<R>*>(a.tw), base + (kRRK + r) * SL);
+  } else {
+    const cplx<R>* tw = team_tw_load<NX, P, R>(base + kRRK * SL, a.tw);
+    vrow_body<R, NX, P>(g, a, grp * 4 + sub * kRRK + r, t / kRRK, base + r * SL, tw);
+  }
 }
 
 // ----------------------------------------------------------------- host side
@@ -116,8 +129,9 @@ static size_t col_lds() {
 
 template <typename R, int N>
 static size_t row_lds() {
-  return ((size_t)kRRK * team_slice(TeamSched<N, vpts_row(N)>::LDS, sizeof(cplx<R>), kRRK) +
-          tw_slots<N, vpts_row(N)>()) * sizeof(cplx<R>);
+  constexpr size_t sl = team_slice(TeamSched<N, vpts_row(N)>::LDS, sizeof(cplx<R>), kRRK);
+  if constexpr (row_pair<R>()) return 2 * kRRK * sl * sizeof(cplx<R>);
+  return (kRRK * sl + tw_slots<N, vpts_row(N)>()) * sizeof(cplx<R>);
 }
 
 template <typename R, int N, int STAGE>

@@ -324,6 +324,14 @@ __device__ __forceinline__ void team_fft2(cplx<R> (&v1)[P], cplx<R> (&v2)[P], cp
   team_passes2<N, P, 0, DIR, NREG, R>(v1, v2, lds1, lds2, tid, tw, t.w);
 }
 
+// Two independent transforms side by side with every twiddle from the LDS table.
+template <int N, int P, int DIR, typename R>
+__device__ __forceinline__ void team_fft2(cplx<R> (&v1)[P], cplx<R> (&v2)[P], cplx<R>* lds1, cplx<R>* lds2, int tid,
+                                          const cplx<R>* tw) {
+  static_assert(TeamSched<N, P>::NPASS >= 2, "team_fft needs at least two passes");
+  team_passes2<N, P, 0, DIR, 0, R>(v1, v2, lds1, lds2, tid, tw, nullptr);
+}
+
 // The same transform with the first t.NR passes' twiddles from registers
 // (twr_fill) and the rest from the LDS table tw.
 template <int N, int P, int DIR, typename R, int NREG>

@@ -65,6 +65,14 @@ struct VortAdjRowArgs {
 };
 
 // STAGE 0: entry from LAM.  STAGE 1..4: after dN(c)^T, dN(b)^T, dN(a)^T, dN(u)^T.
+#ifndef FF_ACOL_PAIR
+#define FF_ACOL_PAIR 1  // FP32 adjoint column pass up to 1024: paired transforms
+#endif
+template <typename R, int N>
+constexpr bool acol_pair() {
+  return sizeof(R) == 4 && N <= 1024 && FF_ACOL_PAIR;
+}
+
 template <typename R, int NY, int P, int STAGE>
 __global__ void __launch_bounds__(kCBK*(NY / P)) k_vacol(VortGeom g, VortAdjColArgs a) {
   constexpr int T = NY / P;
@@ -78,8 +86,10 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vacol(VortGeom g, VortAdjColA
   const int c = sub * kCBK + cc;
   const int tid = t / kCBK;
   constexpr int SL = team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), kCBK);
+  constexpr bool pair = acol_pair<R, NY>();  // second buffers after the first, then the table
   cplx<R>* lds = reinterpret_cast<cplx<R>*>(vsm) + cc * SL;
-  const cplx<R>* tw = team_tw_load<NY, P, R>(reinterpret_cast<cplx<R>*>(vsm) + kCBK * SL, a.tw);
+  cplx<R>* lds2 = pair ? reinterpret_cast<cplx<R>*>(vsm) + (kCBK + cc) * SL : nullptr;
+  const cplx<R>* tw = team_tw_load<NY, P, R>(reinterpret_cast<cplx<R>*>(vsm) + (pair ? 2 : 1) * kCBK * SL, a.tw);
   const int ix = tx * CB + c;
   const bool colkept = ix < g.KX;
   const R kx = (R)(g.dk * (double)ix);
@@ -93,8 +103,27 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vacol(VortGeom g, VortAdjColA
                            reinterpret_cast<const cplx<R>*>(a.OA2)};
 #pragma unroll
     for (int p = 0; p < P; ++p) mb[p] = mkc<R>(0, 0);
+    if constexpr (pair) {  // U and V side by side, then Z
+      cplx<R> v0[P], v1[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        v0[p] = O[0][ibase + (uint32_t)(tid + p * T) * CB];
+        v1[p] = O[1][ibase + (uint32_t)(tid + p * T) * CB];
+      }
+      team_fft2<NY, P, -1, R>(v0, v1, lds, lds2, tid, tw);
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const int iy = tid + p * T;
+        const R l = (R)(g.dl * (double)vfreq(iy, NY));
+        const R K2 = kx * kx + l * l;
+        const R invK = (ix == 0 && iy == 0) ? R(0) : R(1) / K2;
+        const R q0 = l * invK, q1 = -kx * invK;
+        mb[p] = mkc<R>(mb[p].x + q0 * v0[p].y, mb[p].y - q0 * v0[p].x);
+        mb[p] = mkc<R>(mb[p].x + q1 * v1[p].y, mb[p].y - q1 * v1[p].x);
+      }
+    }
 #pragma unroll 1
-    for (int f = 0; f < 3; ++f) {
+    for (int f = pair ? 2 : 0; f < 3; ++f) {
       cplx<R> v[P];
 #pragma unroll
       for (int p = 0; p < P; ++p) v[p] = O[f][ibase + (uint32_t)(tid + p * T) * CB];
@@ -187,6 +216,23 @@ __global__ void __launch_bounds__(kCBK*(NY / P)) k_vacol(VortGeom g, VortAdjColA
   // inverse y-FFTs of i kx s / W and i l s / W (W = 1 at kx = 0, 2 otherwise)
   const R invW = ix == 0 ? R(1) : R(0.5);
   cplx<R>* dst[2] = {reinterpret_cast<cplx<R>*>(a.IA0), reinterpret_cast<cplx<R>*>(a.IA1)};
+  if constexpr (pair) {
+    cplx<R> v0[P], v1[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int iy = tid + p * T;
+      const R q0 = kx * invW, q1 = (R)(g.dl * (double)vfreq(iy, NY)) * invW;
+      v0[p] = mkc<R>(-q0 * nx_[p].y, q0 * nx_[p].x);  // i q s
+      v1[p] = mkc<R>(-q1 * nx_[p].y, q1 * nx_[p].x);
+    }
+    team_fft2<NY, P, 1, R>(v0, v1, lds, lds2, tid, tw);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      dst[0][ibase + (uint32_t)(tid + p * T) * CB] = v0[p];
+      dst[1][ibase + (uint32_t)(tid + p * T) * CB] = v1[p];
+    }
+    return;
+  }
 #pragma unroll 1
   for (int f = 0; f < 2; ++f) {
     cplx<R> v[P];
@@ -288,8 +334,8 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_varow(VortGeom g, VortAdjRowA
 template <typename R, int N, int STAGE>
 static int launch_acol(const VortGeom& g, const VortAdjColArgs& a, hipStream_t st) {
   constexpr int P = vpts_col(N);
-  const size_t sh =
-      ((size_t)kCBK * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), kCBK) + tw_slots<N, P>()) * sizeof(cplx<R>);
+  constexpr size_t sl = team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), kCBK);
+  const size_t sh = ((acol_pair<R, N>() ? 2 : 1) * kCBK * sl + tw_slots<N, P>()) * sizeof(cplx<R>);
   auto kern = k_vacol<R, N, P, STAGE>;
   static bool attr_set = false;
   if (!attr_set) {

@@ -82,9 +82,11 @@ __device__ __forceinline__ cplx<R> vfield(int f, const cplx<R> z, const R kx, co
 // threads (team index tid) with its own LDS slice.  live = false marks a
 // padding column beyond the last tile: it joins the team barriers but reads
 // zeros and writes nothing.
-template <typename R, int NY, int P, int STAGE>
+// PAIR: independent transforms run side by side (team_fft2) in lds and lds2.
+template <typename R, int NY, int P, int STAGE, bool PAIR = false>
 __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& a, const int tx, const int c,
-                                          const int tid, cplx<R>* lds, const cplx<R>* tw, const bool live) {
+                                          const int tid, cplx<R>* lds, const cplx<R>* tw, const bool live,
+                                          cplx<R>* lds2 = nullptr) {
   constexpr int T = NY / P;
   constexpr int CB = kVCB;
   const int ix = tx * CB + c;
@@ -117,10 +119,17 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
     const cplx<R>* O1 = reinterpret_cast<const cplx<R>*>(a.O1);
 #pragma unroll
     for (int p = 0; p < P; ++p) nn[p] = live ? at32(O0, ibase + (uint32_t)(tid + p * T) * CB) : mkc<R>(0, 0);
-    fft(nn, Fwd{});
+    if constexpr (!PAIR) fft(nn, Fwd{});
 #pragma unroll
     for (int p = 0; p < P; ++p) vb[p] = live ? at32(O1, ibase + (uint32_t)(tid + p * T) * CB) : mkc<R>(0, 0);
-    fft(vb, Fwd{});
+    if constexpr (PAIR) {
+      if constexpr (treg)
+        team_fft2<NY, P, -1, R>(nn, vb, lds, lds2, tid, tw, twr);
+      else
+        team_fft2<NY, P, -1, R>(nn, vb, lds, lds2, tid, tw);
+    } else {
+      fft(vb, Fwd{});
+    }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       const int iy = tid + p * T;
@@ -146,6 +155,33 @@ __device__ __forceinline__ void vcol_body(const VortGeom& g, const VortColArgs& 
   // inverse y-FFTs of zeta^, u^ = i l/|k|^2 zeta^, v^ = -i kr/|k|^2 zeta^
   cplx<R>* dst[3] = {reinterpret_cast<cplx<R>*>(a.I0), reinterpret_cast<cplx<R>*>(a.I1),
                      reinterpret_cast<cplx<R>*>(a.I2)};
+  if constexpr (PAIR) {  // u^ and v^ side by side, then zeta^ (the last use of nw)
+    cplx<R> v1[P], v2[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int iy = tid + p * T;
+      const R l = (R)(g.dl * (double)vfreq(iy, NY));
+      v1[p] = vfield<R>(1, nw[p], kx, l, ix == 0 && iy == 0);
+      v2[p] = vfield<R>(2, nw[p], kx, l, ix == 0 && iy == 0);
+    }
+    if constexpr (treg)
+      team_fft2<NY, P, 1, R>(v1, v2, lds, lds2, tid, tw, twr);
+    else
+      team_fft2<NY, P, 1, R>(v1, v2, lds, lds2, tid, tw);
+    if (live) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        at32(dst[1], ibase + (uint32_t)(tid + p * T) * CB) = v1[p];
+        at32(dst[2], ibase + (uint32_t)(tid + p * T) * CB) = v2[p];
+      }
+    }
+    fft(nw, Inv{});
+    if (live) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) at32(dst[0], ibase + (uint32_t)(tid + p * T) * CB) = nw[p];
+    }
+    return;
+  }
 #pragma unroll 1
   for (int f = 0; f < 3; ++f) {
     cplx<R> v[P];

@@ -33,6 +33,14 @@
 namespace ffhip {
 
 
+#ifndef FF_COL_PAIR
+#define FF_COL_PAIR 1  // FP32 column pass up to 1024: paired transforms (LDS has room there)
+#endif
+template <typename R, int N>
+constexpr bool col_pair() {
+  return sizeof(R) == 4 && N <= 1024 && FF_COL_PAIR;
+}
+
 #ifndef FF_ROW_PAIR
 #define FF_ROW_PAIR 1  // FP32 single-grid row pass: the two inverse transforms side by side
 #endif
@@ -59,9 +67,11 @@ __global__ void __launch_bounds__(kCBK*(NY / P), sizeof(R) == 4 ? FF_COL_WAVES :
   const int t = threadIdx.x;
   const int cc = t % kCBK;
   constexpr int SL = team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), kCBK);
+  constexpr bool pair = col_pair<R, NY>();  // second buffers after the first, then the table
   cplx<R>* base = reinterpret_cast<cplx<R>*>(vsm);
-  const cplx<R>* tw = team_tw_load<NY, P, R>(base + kCBK * SL, a.tw);
-  vcol_body<R, NY, P, STAGE>(g, a, tx, sub * kCBK + cc, t / kCBK, base + cc * SL, tw, true);
+  const cplx<R>* tw = team_tw_load<NY, P, R>(base + (pair ? 2 : 1) * kCBK * SL, a.tw);
+  vcol_body<R, NY, P, STAGE, pair>(g, a, tx, sub * kCBK + cc, t / kCBK, base + cc * SL, tw, true,
+                                   pair ? base + (kCBK + cc) * SL : nullptr);
 }
 
 // One column per workgroup of three teams (vcol3_body); the four columns of a
@@ -123,8 +133,8 @@ static bool pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 template <typename R, int N>
 static size_t col_lds() {
-  return ((size_t)kCBK * team_slice(TeamSched<N, vpts_col(N)>::LDS, sizeof(cplx<R>), kCBK) +
-          tw_slots<N, vpts_col(N)>()) * sizeof(cplx<R>);
+  constexpr size_t sl = team_slice(TeamSched<N, vpts_col(N)>::LDS, sizeof(cplx<R>), kCBK);
+  return ((col_pair<R, N>() ? 2 : 1) * kCBK * sl + tw_slots<N, vpts_col(N)>()) * sizeof(cplx<R>);
 }
 
 template <typename R, int N>

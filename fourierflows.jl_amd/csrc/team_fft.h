@@ -102,7 +102,7 @@ constexpr int tw_slots() {
   return tw_off<N, P, TeamSched<N, P>::NPASS>();
 }
 // the same count for sizes known at run time (power-of-two n, points p)
-inline int tw_slots_rt(int n, int p) {
+constexpr int tw_slots_rt(int n, int p) {
   int total = 0, ns = 1, rest = n;
   while (rest % p == 0 && rest >= p) {
     if (ns > 1) total += (p - 1) * ns;
@@ -112,6 +112,13 @@ inline int tw_slots_rt(int n, int p) {
   if (rest > 1 && ns > 1) total += (rest - 1) * ns;
   return total;
 }
+
+static_assert(tw_slots<2048, 16>() == tw_slots_rt(2048, 16) && tw_slots<1024, 16>() == tw_slots_rt(1024, 16) &&
+                  tw_slots<512, 16>() == tw_slots_rt(512, 16) && tw_slots<256, 16>() == tw_slots_rt(256, 16) &&
+                  tw_slots<1024, 8>() == tw_slots_rt(1024, 8) && tw_slots<512, 8>() == tw_slots_rt(512, 8) &&
+                  tw_slots<256, 8>() == tw_slots_rt(256, 8) && tw_slots<128, 8>() == tw_slots_rt(128, 8) &&
+                  tw_slots<64, 8>() == tw_slots_rt(64, 8),
+              "run-time twiddle table size must match the compile-time one");
 
 template <int N, int P, int S, typename R>
 __device__ __forceinline__ void tw_fill_pass(cplx<R>* ltw, const cplx<R>* roots, int t, int nt) {

@@ -64,6 +64,26 @@ def test_fused_fp64_matches_oracle(ff, n, ny, Ly):
     assert err < 1e-12, err
 
 
+@pytest.mark.parametrize("n,ny,Ly", [(64, 64, 2 * np.pi), (256, 128, 4 * np.pi), (128, 512, 2 * np.pi),
+                                     (512, 512, 2 * np.pi), (1024, 1024, 2 * np.pi), (2048, 1024, np.pi),
+                                     (1024, 2048, 2 * np.pi)])
+def test_fused_fp32_matches_oracle(ff, n, ny, Ly):
+    """The FP32 fused passes (paired transforms up to 1024 in the column pass,
+    paired inverses in every row pass) against the FP64 oracle: relative L2
+    at FP32 roundoff after the steps."""
+    o = og.TwoDGrid(n, 2 * np.pi, ny, Ly, x0=0, y0=0)
+    zh0 = oics.vorticity_ic(o, seed=12, k0=min(6.0, n / 10))
+    L = oph.vorticity_linear(o, nu=2e-3)
+    steps = 30 if n * ny <= 512 * 512 else 6
+    ref = _oracle(n, ny, Ly, zh0, L, 0.004, steps)
+    prob = ff.TwoDVorticityProblem(nx=n, ny=ny, Ly=Ly, nu=2e-3, dt=0.004, T=np.float32)
+    assert prob.eqn.calcN.fused
+    prob.sol.copy_(_dev(zh0.astype(np.complex64)))
+    ff.stepforward(prob, steps)
+    err = ou.rel_l2(_host(prob.sol).astype(np.complex128), ref)
+    assert err < 1e-5, err
+
+
 @pytest.mark.parametrize("prec", [64, 32])
 def test_fused_matches_unfused(ff, monkeypatch, prec):
     T = np.float64 if prec == 64 else np.float32

@@ -13,6 +13,8 @@ include/ffhip.h) that keeps the reference's API names:
 * ``parsevalsum2``, ``parsevalsum``               (src/utils.jl)
 * ``Diagnostic``, ``increment``, ``update`` and ``stepforward(prob, diags,
   nsteps)``                                        (src/diagnostics.jl)
+* ``Output``, ``saveproblem``, ``saveoutput``, ``savediagnostic`` (npz
+  archives with the reference's group layout)     (src/output.jl)
 * built-in operators ``VorticityOperator`` / ``TracerOperator`` and the
   ``Diffusion`` testbed                              (src/diffusion.jl)
 * ``adjoint``: differentiable ``ldiv``/``mul``/``dealias`` and the discrete
@@ -22,8 +24,9 @@ Device arrays are torch tensors on the ROCm device (``HIP()`` is the device
 tag, the analogue of ``GPU()``).
 """
 
-from . import _lib, adjoint, diagnostics
+from . import _lib, adjoint, diagnostics, output
 from .diagnostics import Diagnostic, increment, update
+from .output import Output, savediagnostic, saveoutput, saveproblem
 from ._lib import DomainError, FFHIPError, NonFiniteError
 from .core import CPU, GPU, HIP, cxtype, device_array, fltype, init, zeros
 from .domains import (OneDGrid, Plan, ThreeDGrid, TwoDGrid, dealias, getaliasedwavenumbers, gridpoints, ldiv,

@@ -14,6 +14,9 @@
 // so consecutive lanes read consecutive slots.  Reading them from global
 // memory instead put an L2 round trip into every pass and, since stores and
 // loads share one counter, made each pass wait for the stores before it.
+// Kernels that run several transforms of one size can keep a thread's
+// factors in registers instead (TwRegs), and two independent transforms can
+// run side by side through one set of barriers (team_fft2).
 #pragma once
 
 #include "fft_device.h"
@@ -137,7 +140,7 @@ __device__ __forceinline__ void tw_fill_pass(cplx<R>* ltw, const cplx<R>* roots,
 }
 
 // Fill the pass-ordered table from the N roots of unity (all threads of the
-// block; the caller's barrier publishes it).  Returns ltw.
+// block take part; the block barrier at the end publishes it).  Returns ltw.
 template <int N, int P, typename R>
 __device__ __forceinline__ const cplx<R>* team_tw_load(cplx<R>* ltw, const void* roots) {
   tw_fill_pass<N, P, 1, R>(ltw, reinterpret_cast<const cplx<R>*>(roots), (int)threadIdx.x, (int)blockDim.x);

@@ -66,6 +66,17 @@ __device__ __forceinline__ const T& at32(const T* base, uint32_t idx) {
 constexpr int vpts_col(int n) { return n >= 256 ? 16 : 8; }
 constexpr int vpts_row(int n) { return n >= 256 ? 16 : 8; }
 
+// Points per thread of the single-team column passes (forward and
+// adjoint).  FP32 takes 8 instead of 16 where that adds no LDS exchange pass
+// (512 = 8^3): twice the threads per column for the same passes and fewer
+// registers (measured on MI355X: 512^2 forward 46.0 -> 37.4 ms per 500
+// steps).  Where 8 adds a pass (1024 and 2048: 4 vs 3) it stays at 16
+// (2048^2 measured 34.5 vs 27.5 us per stage).
+template <typename R, int N>
+constexpr int vcol_pts() {
+  return sizeof(R) == 4 && N >= 256 && TeamSched<N, 8>::NPASS == TeamSched<N, 16>::NPASS ? 8 : vpts_col(N);
+}
+
 struct VortGeom {
   int nx, ny, nkr;
   int KX, KXp, TX;  // kept kx count, padded to CB, column blocks

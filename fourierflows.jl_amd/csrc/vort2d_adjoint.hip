@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(kRRK*(NX / P)) k_varow(VortGeom g, VortAdjRowA
 // ----------------------------------------------------------------- launchers
 template <typename R, int N, int STAGE>
 static int launch_acol(const VortGeom& g, const VortAdjColArgs& a, hipStream_t st) {
-  constexpr int P = vpts_col(N);
+  constexpr int P = vcol_pts<R, N>();
   constexpr size_t sl = team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), kCBK);
   const size_t sh = ((acol_pair<R, N>() ? 2 : 1) * kCBK * sl + tw_slots<N, P>()) * sizeof(cplx<R>);
   auto kern = k_vacol<R, N, P, STAGE>;

@@ -133,8 +133,8 @@ static bool pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 template <typename R, int N>
 static size_t col_lds() {
-  constexpr size_t sl = team_slice(TeamSched<N, vpts_col(N)>::LDS, sizeof(cplx<R>), kCBK);
-  return ((col_pair<R, N>() ? 2 : 1) * kCBK * sl + tw_slots<N, vpts_col(N)>()) * sizeof(cplx<R>);
+  constexpr size_t sl = team_slice(TeamSched<N, vcol_pts<R, N>()>::LDS, sizeof(cplx<R>), kCBK);
+  return ((col_pair<R, N>() ? 2 : 1) * kCBK * sl + tw_slots<N, vcol_pts<R, N>()>()) * sizeof(cplx<R>);
 }
 
 template <typename R, int N>
@@ -146,7 +146,7 @@ static size_t row_lds() {
 
 template <typename R, int N, int STAGE>
 static int launch_col(const VortGeom& g, const VortColArgs& a, hipStream_t st) {
-  constexpr int P = vpts_col(N);
+  constexpr int P = vcol_pts<R, N>();
   const size_t sh = col_lds<R, N>();
   auto kern = k_vcol<R, N, P, STAGE>;
   static bool attr_set = false;

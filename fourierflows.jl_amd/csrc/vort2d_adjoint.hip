@@ -359,7 +359,7 @@ static int launch_acol(const VortGeom& g, const VortAdjColArgs& a, hipStream_t s
 
 template <typename R, int N>
 static int launch_arow(const VortGeom& g, const VortAdjRowArgs& a, hipStream_t st) {
-  constexpr int P = vpts_row(N);
+  constexpr int P = vrow_pts<R, N>();
   const size_t sh =
       ((size_t)kRRK * team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), kRRK) + tw_slots<N, P>()) * sizeof(cplx<R>);
   auto kern = k_varow<R, N, P>;

@@ -139,9 +139,9 @@ static size_t col_lds() {
 
 template <typename R, int N>
 static size_t row_lds() {
-  constexpr size_t sl = team_slice(TeamSched<N, vpts_row(N)>::LDS, sizeof(cplx<R>), kRRK);
+  constexpr size_t sl = team_slice(TeamSched<N, vrow_pts<R, N>()>::LDS, sizeof(cplx<R>), kRRK);
   if constexpr (row_pair<R>()) return 2 * kRRK * sl * sizeof(cplx<R>);
-  return (kRRK * sl + tw_slots<N, vpts_row(N)>()) * sizeof(cplx<R>);
+  return (kRRK * sl + tw_slots<N, vrow_pts<R, N>()>()) * sizeof(cplx<R>);
 }
 
 template <typename R, int N, int STAGE>
@@ -220,7 +220,7 @@ static int launch_row3(const VortGeom& g, const VortRowArgs& a, hipStream_t st) 
 
 template <typename R, int N>
 static int launch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
-  constexpr int P = vpts_row(N);
+  constexpr int P = vrow_pts<R, N>();
   const size_t sh = row_lds<R, N>();
   auto kern = k_vrow<R, N, P>;
   static bool attr_set = false;

@@ -43,5 +43,6 @@ int fused_forward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void* s
                        int64_t ncheck, hipStream_t st);
 int fused_adjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, void* lam0, int64_t nsteps, int64_t ncheck,
                   hipStream_t st);
+int64_t fused_checkpoint_id(const ffhip_op* op);
 
 }  // namespace ffhip

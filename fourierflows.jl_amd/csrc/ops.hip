@@ -680,6 +680,13 @@ int ffhip_op_etdrk4_adjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, vo
   return fused_adjoint(op, ts, lamT, lam0, nsteps, ncheck, as_stream(stream));
 }
 
+int ffhip_op_checkpoint_id(const ffhip_op* op, int64_t* id) {
+  clear_error();
+  FF_CHECK(op && id, FFHIP_ERR_ARG, "null argument");
+  *id = op->fused ? fused_checkpoint_id(op) : 0;
+  return FFHIP_OK;
+}
+
 int ffhip_op_rk4_adjoint(ffhip_op* op, ffhip_rk4* ts, const void* lamT, void* lam0, double dt, int64_t nsteps,
                          void* stream) {
   clear_error();

@@ -116,6 +116,7 @@ struct VortState {
   // the checkpointed forward run the checkpoints belong to
   const void* ck_ts = nullptr;
   int64_t ck_nsteps = -1, ck_ncheck = 0;
+  int64_t ck_id = 0;     // identifies that run (0: none); each forward_ckpt call takes a new one
   bool fs_last = false;  // FS holds the last segment's fields from the forward run
 };
 

@@ -25,6 +25,7 @@
 //   bb = dN(b)^T N3b; ub += E/2 bb; N2b += zeta bb
 //   ab += dN(a)^T N2b; ub += E/2 ab; N1b += zeta ab
 //   lam_prev = ub + dN(u)^T N1b
+#include <atomic>
 #include <cstring>
 #include <new>
 
@@ -444,6 +445,9 @@ static int ensure_adjoint_buffers(VortState* s, int prec, int64_t ncheck, int64_
   return FFHIP_OK;
 }
 
+// run identifiers of checkpointed forward runs, unique over all operators of the process
+static std::atomic<int64_t> g_ck_seq{0};
+
 struct VortAdjPlan {
   int64_t nsteps = 0, ncheck = 0, seg = 0;
 };
@@ -508,6 +512,7 @@ static int vforward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void*
   const int64_t seg = (nsteps + ncheck - 1) / ncheck;
   s->ck_ts = nullptr;
   s->ck_nsteps = -1;
+  s->ck_id = 0;
   FF_TRY(ensure_adjoint_buffers(s, prec, ncheck, seg));
   FF_TRY(vort_ensure_packed(op, ts, g, st));
   const size_t n = (size_t)g.TX * g.KY * kVCB;
@@ -533,6 +538,7 @@ static int vforward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void*
   s->ck_ts = ts;
   s->ck_nsteps = nsteps;
   s->ck_ncheck = ncheck;
+  s->ck_id = ++g_ck_seq;
   return FFHIP_OK;
 }
 
@@ -621,6 +627,11 @@ int fused_forward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol0, void* s
   FF_CHECK(op->fused, FFHIP_ERR_UNSUPPORTED, "the adjoint needs the fused path (power-of-two 2D vorticity grid)");
   return op->prec == FFHIP_F32 ? vforward_ckpt<float>(op, ts, sol0, solT, nsteps, ncheck, st)
                                : vforward_ckpt<double>(op, ts, sol0, solT, nsteps, ncheck, st);
+}
+
+int64_t fused_checkpoint_id(const ffhip_op* op) {
+  const VortState* s = reinterpret_cast<const VortState*>(op->fused);
+  return s ? s->ck_id : 0;
 }
 
 int fused_adjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, void* lam0, int64_t nsteps, int64_t ncheck,

@@ -98,6 +98,7 @@ SIGNATURES = {
     "ffhip_op_is_fused": [_vp, C.POINTER(_int)],
     "ffhip_op_etdrk4_forward_ckpt": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
     "ffhip_op_etdrk4_adjoint": [_vp, _vp, _vp, _vp, _i64, _i64, _vp],
+    "ffhip_op_checkpoint_id": [_vp, _vp],
     "ffhip_op_rk4_adjoint": [_vp, _vp, _vp, _vp, _dbl, _i64, _vp],
     "ffhip_op_explicit_run": [_vp, _vp, _int, _vp, _dbl, _i64, _i64, _vp, _vp, _vp],
     "ffhip_etdrk4_set_filter": [_vp, _dp],

@@ -85,23 +85,43 @@ class _Dealias:
         return None, out
 
 
+def _checkpoint_id(op):
+    import ctypes
+
+    v = ctypes.c_int64(0)
+    _lib.call("ffhip_op_checkpoint_id", op.handle, ctypes.byref(v))
+    return v.value
+
+
 class _ETDRK4Solve:
     @staticmethod
     def forward(ctx, prob, sol0, nsteps, ncheck):
         op = _fused_operator(prob)
         ts = prob.timestepper
-        solT = sol0.detach().clone().contiguous()
+        sol0 = sol0.detach().contiguous()
+        solT = sol0.clone()
         _lib.call("ffhip_op_etdrk4_forward_ckpt", op.handle, ts.handle, ptr(solT), ptr(solT), int(nsteps),
                   int(ncheck), stream_ptr())
         ctx.prob, ctx.nsteps, ctx.ncheck = prob, int(nsteps), int(ncheck)
+        # the operator holds one set of checkpoints: remember which run made
+        # them and keep sol0, so that backward can rebuild them if a later
+        # solve on the same operator has replaced them
+        ctx.sol0 = sol0.clone()
+        ctx.ck_id = _checkpoint_id(op)
         return solT
 
     @staticmethod
     def backward(ctx, g):
         prob = ctx.prob
         op = _fused_operator(prob)
+        ts = prob.timestepper
+        if _checkpoint_id(op) != ctx.ck_id:
+            scratch = ctx.sol0.clone()
+            _lib.call("ffhip_op_etdrk4_forward_ckpt", op.handle, ts.handle, ptr(scratch), ptr(scratch), ctx.nsteps,
+                      ctx.ncheck, stream_ptr())
+            ctx.ck_id = _checkpoint_id(op)
         lam = g.detach().clone().contiguous()
-        _lib.call("ffhip_op_etdrk4_adjoint", op.handle, prob.timestepper.handle, ptr(lam), ptr(lam), ctx.nsteps,
+        _lib.call("ffhip_op_etdrk4_adjoint", op.handle, ts.handle, ptr(lam), ptr(lam), ctx.nsteps,
                   ctx.ncheck, stream_ptr())
         return None, lam, None, None
 

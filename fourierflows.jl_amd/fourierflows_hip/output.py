@@ -15,15 +15,23 @@ it returns, as the reference does.  ``saveoutput(out, background=True)``
 instead starts the device-to-host copies on the current stream into pinned
 host buffers and leaves the file write to a writer thread, so the caller can
 launch the next steps at once; ``out.flush()`` waits for pending writes.
+
+Every write to one path holds that path's lock, and a synchronous write
+(``saveoutput``, ``saveproblem``, ``savediagnostic``) first waits for the
+background snapshots already queued for its path, so the archive receives
+whole members in call order.  Writers still pending when the interpreter
+exits are flushed and closed then.
 """
 
 from __future__ import annotations
 
+import atexit
 import io
 import os
 import queue
 import threading
 import types
+import weakref
 import zipfile
 
 import numpy as np
@@ -63,18 +71,60 @@ def _npy_bytes(arr):
     return buf.getvalue()
 
 
+_GUARD = threading.Lock()
+_LOCKS = {}    # absolute path -> lock held by every write to that file
+_WRITERS = {}  # absolute path -> weak references to Outputs with a background writer
+
+
+def _lock_for(path):
+    key = os.path.abspath(path)
+    with _GUARD:
+        lk = _LOCKS.get(key)
+        if lk is None:
+            lk = _LOCKS[key] = threading.Lock()
+    return lk
+
+
+def _pending_writers(path):
+    key = os.path.abspath(path)
+    with _GUARD:
+        refs = _WRITERS.get(key, [])
+        outs = [r() for r in refs]
+        _WRITERS[key] = [r for r, o in zip(refs, outs) if o is not None]
+    return [o for o in outs if o is not None]
+
+
+def _flush_pending(path):
+    """Wait for the background snapshots queued for ``path`` (in call order)."""
+    for o in _pending_writers(path):
+        o.flush()
+
+
+@atexit.register
+def _close_all():
+    with _GUARD:
+        outs = [r() for refs in _WRITERS.values() for r in refs]
+    for o in outs:
+        if o is not None:
+            try:
+                o.close()
+            except Exception:  # the process is ending; nothing left to report to
+                pass
+
+
 def _write(path, items):
     """Append (key, host value) pairs; a key already in the file is an error, as
     a second write to one JLD2 dataset is."""
-    mode = "a" if os.path.isfile(path) else "w"
-    with zipfile.ZipFile(path, mode, compression=zipfile.ZIP_STORED) as z:
-        have = set(z.namelist())
-        for key, value in items:
-            name = key + ".npy"
-            if name in have:
-                raise KeyError(f"{key!r} is already in {path}")
-            z.writestr(name, _npy_bytes(value))
-            have.add(name)
+    with _lock_for(path):
+        mode = "a" if os.path.isfile(path) else "w"
+        with zipfile.ZipFile(path, mode, compression=zipfile.ZIP_STORED) as z:
+            have = set(z.namelist())
+            for key, value in items:
+                name = key + ".npy"
+                if name in have:
+                    raise KeyError(f"{key!r} is already in {path}")
+                z.writestr(name, _npy_bytes(value))
+                have.add(name)
 
 
 def load(path):
@@ -136,6 +186,8 @@ class Output:
             self._queue = queue.Queue()
             self._thread = threading.Thread(target=self._worker, daemon=True)
             self._thread.start()
+            with _GUARD:
+                _WRITERS.setdefault(os.path.abspath(self.path), []).append(weakref.ref(self))
         self._queue.put((event, items))
 
     def flush(self):
@@ -163,6 +215,7 @@ def saveoutput(out, background=False):
     if not background:
         for name in out.fields:
             items.append((f"snapshots/{name}/{step}", _host(out[name])))
+        _flush_pending(out.path)
         _write(out.path, items)
         return None
     import torch
@@ -227,6 +280,7 @@ def saveproblem(prob, filename=None):
     items.append(("clock/dt", np.float64(prob.clock.dt)))
     items += _grid_items(prob.grid)
     items += _param_items(prob.params)
+    _flush_pending(filename)
     _write(filename, items)
 
 
@@ -235,6 +289,7 @@ def savediagnostic(diagnostic, diagname, filename):
     and data up to the diagnostic's last saved entry."""
     i = diagnostic.i
     data = [_host(v) for v in diagnostic.data[:i]]
+    _flush_pending(filename)
     _write(filename, [(f"diagnostics/{diagname}/steps", np.asarray(diagnostic.steps[:i], dtype=np.int64)),
                       (f"diagnostics/{diagname}/t", np.asarray(diagnostic.t[:i], dtype=np.float64)),
                       (f"diagnostics/{diagname}/data", np.stack(data) if data else np.zeros(0))])

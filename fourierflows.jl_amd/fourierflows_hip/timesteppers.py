@@ -58,10 +58,22 @@ class Equation:
 
 
 class Clock:
-    """src/problem.jl:43-50."""
+    """src/problem.jl:43-50: ``Clock{T}`` -- ``dt`` and ``t`` are scalars of the
+    grid's float type ``T`` and every step adds ``dt`` to ``t`` once in that
+    type (src/timesteppers.jl:542-545), also when the library takes many steps
+    in one call."""
 
-    def __init__(self, dt, t=0.0, step=0):
-        self.dt, self.t, self.step = dt, t, step
+    def __init__(self, dt, t=0.0, step=0, T=np.float64):
+        T = np.dtype(T).type
+        self.dt, self.t, self.step = T(dt), T(t), step
+
+    def advance(self, nsteps):
+        """``clock.t += clock.dt; clock.step += 1``, ``nsteps`` times."""
+        t, dt = self.t, self.dt
+        for _ in range(int(nsteps)):
+            t = t + dt
+        self.t = t
+        self.step += int(nsteps)
 
     def __repr__(self):
         return f"Clock(dt={self.dt}, step={self.step}, t={self.t})"
@@ -123,13 +135,11 @@ class ETDRK4TimeStepper(_Handle):
         self.substep3(self.sol2, self.sol1, self.N1, self.N3)
         eqn.calcN(self.N4, self.sol2, clock.t + clock.dt, clock, vars, params, grid)
         self.update(sol, self.N1, self.N2, self.N3, self.N4)
-        clock.t += clock.dt
-        clock.step += 1
+        clock.advance(1)
 
     def run_fused(self, op, sol, clock, nsteps):
         _lib.call("ffhip_op_etdrk4_run", op.handle, self._h, ptr(sol), int(nsteps), stream_ptr())
-        clock.t += nsteps * clock.dt
-        clock.step += nsteps
+        clock.advance(nsteps)
 
 
 class RK4TimeStepper(_Handle):
@@ -179,13 +189,11 @@ class RK4TimeStepper(_Handle):
         eqn.calcN(self.RHS4, self.sol1, t + dt, clock, vars, params, grid)
         self.addlinearterm(self.RHS4, self.sol1)
         self.update(sol, dt)
-        clock.t += clock.dt
-        clock.step += 1
+        clock.advance(1)
 
     def run_fused(self, op, sol, clock, nsteps):
         _lib.call("ffhip_op_rk4_run", op.handle, self._h, ptr(sol), float(clock.dt), int(nsteps), stream_ptr())
-        clock.t += nsteps * clock.dt
-        clock.step += nsteps
+        clock.advance(nsteps)
 
 
 def _attach_filter(ts, eqn, setter, filterkw):
@@ -218,8 +226,7 @@ class FilteredETDRK4TimeStepper(ETDRK4TimeStepper):
         eqn.calcN(self.N4, self.sol2, clock.t + clock.dt, clock, vars, params, grid)
         self.update(sol, self.N1, self.N2, self.N3, self.N4)
         _lib.call("ffhip_etdrk4_apply_filter", self._h, ptr(sol), stream_ptr())
-        clock.t += clock.dt
-        clock.step += 1
+        clock.advance(1)
 
 
 class FilteredRK4TimeStepper(RK4TimeStepper):
@@ -262,8 +269,7 @@ class _ExplicitBase(RK4TimeStepper):
         _lib.call("ffhip_op_explicit_run", op.handle, self._h, self.kind, ptr(sol), float(clock.dt), int(nsteps),
                   int(clock.step), ptr(m1) if m1 is not None else None, ptr(m2) if m2 is not None else None,
                   stream_ptr())
-        clock.t += nsteps * clock.dt
-        clock.step += nsteps
+        clock.advance(nsteps)
 
 
 class ForwardEulerTimeStepper(_ExplicitBase):
@@ -278,8 +284,7 @@ class ForwardEulerTimeStepper(_ExplicitBase):
     def stepforward(self, sol, clock, eqn, vars, params, grid):
         eqn.calcN(self.N, sol, clock.t, clock, vars, params, grid)
         _lib.call("ffhip_euler_update", self._h, ptr(sol), ptr(self.N), float(clock.dt), stream_ptr())
-        clock.t += clock.dt
-        clock.step += 1
+        clock.advance(1)
 
 
 class AB3TimeStepper(_ExplicitBase):
@@ -299,8 +304,7 @@ class AB3TimeStepper(_ExplicitBase):
         self.addlinearterm(self.RHS, sol)
         _lib.call("ffhip_ab3_update", self._h, ptr(sol), ptr(self.RHS), ptr(self.RHSm1), ptr(self.RHSm2),
                   float(clock.dt), int(clock.step), stream_ptr())
-        clock.t += clock.dt
-        clock.step += 1
+        clock.advance(1)
         self.RHSm2.copy_(self.RHSm1)
         self.RHSm1.copy_(self.RHS)
 
@@ -342,8 +346,7 @@ class LSRK54TimeStepper(_ExplicitBase):
                       float(dt), stream_ptr())
         if self.filtered:
             _lib.call("ffhip_rk4_apply_filter", self._h, ptr(sol), stream_ptr())
-        clock.t += clock.dt
-        clock.step += 1
+        clock.advance(1)
 
 
 def _filtered(cls, name):
@@ -399,7 +402,7 @@ class Problem:
         self.eqn, self.grid = eqn, grid
         self.vars = EmptyVars() if vars is None else vars
         self.params = EmptyParams() if params is None else params
-        self.clock = Clock(float(np.dtype(fltype(grid.T)).type(dt)))
+        self.clock = Clock(dt, T=fltype(grid.T))
         self.timestepper = TimeStepper(stepper, eqn, dt, dev, **stepperkw)
         self.stepper = stepper
         self.sol = zeros(dev, eqn.T, eqn.dims)
@@ -458,10 +461,11 @@ def step_until(prob, stop_time):
     if not stop_time > prob.clock.t:
         raise ValueError("stop_time must be greater than prob.clock.t")
     dt = prob.clock.dt
-    interval = stop_time - prob.clock.t
+    # a Julia Float64 stop_time promotes the Float32 clock (np.float64 does too)
+    interval = np.float64(stop_time) - prob.clock.t
     nsteps = int(np.floor(interval / dt))
     stepforward(prob, nsteps)
-    t_remaining = interval - prob.clock.t  # as in the reference (:754)
-    prob.clock.dt = t_remaining
+    t_remaining = interval - prob.clock.t  # as in the reference (:752)
+    prob.clock.dt = type(dt)(t_remaining)  # stored in the clock's type, as Clock{T}'s field
     stepforward(prob)
     prob.clock.dt = dt

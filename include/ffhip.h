@@ -241,6 +241,12 @@ int ffhip_op_etdrk4_forward_ckpt(ffhip_op* op, ffhip_etdrk4* ts, const void* sol
                                  int64_t ncheck, void* stream);
 int ffhip_op_etdrk4_adjoint(ffhip_op* op, ffhip_etdrk4* ts, const void* lamT, void* lam0, int64_t nsteps,
                             int64_t ncheck, void* stream);
+/* Identifier of the checkpointed run the operator currently holds (0: none).
+ * Every forward_ckpt call takes a new one, unique in the process, so a
+ * caller that keeps several differentiable solves on one operator (chained
+ * solves, a loss over two solves) can tell whether the checkpoints are still
+ * those of its own run, and re-run forward_ckpt from its saved sol0 if not. */
+int ffhip_op_checkpoint_id(const ffhip_op* op, int64_t* id);
 
 /* Discrete adjoint of nsteps RK4 steps of the tracer operator (no
  * counterpart in the reference): lam0 = (d solT / d sol0)^T lamT for the

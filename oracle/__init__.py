@@ -22,7 +22,9 @@ Modules (each function cites the reference file:line it follows):
 * ``adjoint``      -- discrete adjoint of the ETDRK4/RK4 steps (no
                       counterpart in the reference)
 
-Pinning status of every function is listed in ``oracle/PINNING.md``.
+Every function is pinned by checks in ``tests/test_oracle*.py`` that do not
+restate its formula: the reference's known-answer tests, brute-force DFTs,
+mpmath, closed forms, convergence orders and dot-product identities.
 """
 
 from . import grids, timesteppers, physics, utils, ics  # noqa: F401

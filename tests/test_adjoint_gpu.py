@@ -149,6 +149,52 @@ def test_autograd_matches_explicit(ff):
     assert err < 1e-13, err
 
 
+def test_autograd_chained_solves(ff):
+    """Two solves on one operator before backward: the first solve's
+    checkpoints are replaced by the second's, and its pullback must rebuild
+    them (same nsteps and ncheck, so only the run identifier tells them apart)."""
+    import torch
+
+    n, nsteps = 64, 10
+    prob, z0, zobs = _setup(ff, n, 1e-2, 0.01, 2 * nsteps, np.float64)
+    g = prob.grid
+    # 2 nsteps steps with segments of 5: the same recompute schedule as the chained pair
+    J, grad = ff.adjoint.misfit_gradient(prob, z0, zobs, 2 * nsteps, 4)
+    x = z0.clone().requires_grad_(True)
+    s = ff.adjoint.dealias(g, ff.adjoint.mul(g, x))
+    s1 = ff.adjoint.etdrk4_solve(prob, s, nsteps, 2)
+    s2 = ff.adjoint.etdrk4_solve(prob, s1, nsteps, 2)
+    zT = ff.adjoint.ldiv(g, s2)
+    Ja = 0.5 * torch.sum((zT - zobs) ** 2)
+    Ja.backward()
+    assert abs(float(Ja.detach()) - J) < 1e-12 * J
+    err = float(torch.linalg.vector_norm(x.grad - grad) / torch.linalg.vector_norm(grad))
+    assert err < 1e-12, err
+
+
+def test_autograd_loss_over_two_solves(ff):
+    """J = J(x) + J(2x) through two independent solves: dJ = g(x) + 2 g(2x)."""
+    import torch
+
+    n, nsteps = 64, 12
+    prob, z0, zobs = _setup(ff, n, 1e-2, 0.01, nsteps, np.float64)
+    g = prob.grid
+    Ja_ref, ga = ff.adjoint.misfit_gradient(prob, z0, zobs, nsteps, 3)
+    Jb_ref, gb = ff.adjoint.misfit_gradient(prob, 2 * z0, zobs, nsteps, 3)
+    x = z0.clone().requires_grad_(True)
+    terms = []
+    for scale in (1.0, 2.0):
+        s = ff.adjoint.dealias(g, ff.adjoint.mul(g, scale * x))
+        zT = ff.adjoint.ldiv(g, ff.adjoint.etdrk4_solve(prob, s, nsteps, 3))
+        terms.append(0.5 * torch.sum((zT - zobs) ** 2))
+    (terms[0] + terms[1]).backward()
+    want = ga + 2 * gb
+    assert abs(float(terms[0].detach()) - Ja_ref) < 1e-12 * Ja_ref
+    assert abs(float(terms[1].detach()) - Jb_ref) < 1e-12 * Jb_ref
+    err = float(torch.linalg.vector_norm(x.grad - want) / torch.linalg.vector_norm(want))
+    assert err < 1e-12, err
+
+
 def _taylor(ff, prob, z0, zobs, grad, nsteps, ndir, hs, seed):
     import torch
 

@@ -147,7 +147,7 @@ def test_rfft_random_vs_oracle(ff, shape, prec):
     h = (rng.standard_normal(ref.shape) + 1j * rng.standard_normal(ref.shape)).astype(np.complex128)
     hd = _dev(h.astype(np.result_type(T, np.complex64)))
     ff.ldiv(back, g.rfftplan, hd)
-    assert ou.rel_l2(_host(back), np.fft.irfftn(h, s=shape)) < tol
+    assert ou.rel_l2(_host(back), np.fft.irfftn(h, s=shape, axes=tuple(range(h.ndim - len(shape), h.ndim)))) < tol
 
 
 @pytest.mark.parametrize("shape", [(6, 8), (30, 32), (64, 64, 64)])

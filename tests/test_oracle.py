@@ -436,8 +436,68 @@ def test_vorticity_ic_properties():
     assert abs(np.sqrt(np.mean(z**2)) - 1.0) < 1e-3
     assert np.all(zh[~grids.kept_mask(g)] == 0)
     assert zh[0, 0] == 0
-    # spectrum peaks near K = sqrt(2.5)*k0 for E ∝ K^4 exp(-(K/k0)^2)... check energy-weighted mean K
+
+
+def _shell_fit(g, zh):
+    """Least-squares fit log|zeta^| = a + b log K + c K^2 over the kept modes of
+    the interior kx columns (the irfft/rfft round trip keeps them exactly)."""
     K = np.sqrt(g.Krsq)
-    w = np.abs(zh) ** 2
-    kmean = (K * w).sum() / w.sum()
-    assert 5.0 < kmean < 12.0
+    m = grids.kept_mask(g).copy()
+    m[:, 0] = False
+    m[:, -1] = False
+    m &= K > 0
+    m &= np.abs(zh) > 1e-6 * np.abs(zh).max()  # above the round trip's roundoff
+    A = np.stack([np.ones(m.sum()), np.log(K[m]), K[m] ** 2], axis=1)
+    coef, res, *_ = np.linalg.lstsq(A, np.log(np.abs(zh[m])), rcond=None)
+    resid = np.log(np.abs(zh[m])) - A @ coef
+    return coef, float(np.max(np.abs(resid)))
+
+
+@pytest.mark.parametrize("spectrum,k0", [("k4exp1", 6.0), ("k4exp2", 7.3)])
+def test_vorticity_ic_shell_spectrum(spectrum, k0):
+    """|zeta^|^2 = K E(K) up to one constant: E = K^4 exp(-(K/k0)^2) gives
+    |zeta^| ~ K^(5/2) exp(-K^2/(2 k0^2)), E = K^4 exp(-2 (K/k0)^2) gives
+    exp(-K^2/k0^2).  sqrt(E) in place of sqrt(K E) would fit b = 2, and the
+    other exponential twice (or half) the c; the residual shows the phases
+    carry no amplitude."""
+    g = grids.TwoDGrid(128, 2 * np.pi, x0=0, y0=0)
+    zh = ics.vorticity_ic(g, seed=3, k0=k0, spectrum=spectrum)
+    (a, b, c), resid = _shell_fit(g, zh)
+    assert resid < 1e-8, resid
+    assert abs(b - 2.5) < 1e-8, b
+    want_c = -1 / (2 * k0**2) if spectrum == "k4exp1" else -1 / k0**2
+    assert abs(c - want_c) < 1e-8 * abs(want_c), (c, want_c)
+
+
+@pytest.mark.parametrize("n,sigma", [(32, 0.3), (48, 0.45)])
+def test_tracer_ic_single_gaussian(n, sigma):
+    """One Gaussian: the field is a rank-1 product of 1D profiles whose log has
+    second differences -h^2/sigma^2 (pins the width), and its integral is
+    (2 pi)^(3/2) sigma^3 (pins the unit peak amplitude)."""
+    g = grids.ThreeDGrid(n, 2 * np.pi, x0=0, y0=0, z0=0)
+    c = ics.tracer_ic(g, seed=11, ngauss=1, sigma=sigma)
+    h = 2 * np.pi / n
+    integral = c.sum() * h**3
+    assert abs(integral - (2 * np.pi) ** 1.5 * sigma**3) < 1e-10 * integral
+    iz, iy, ix = np.unravel_index(np.argmax(c), c.shape)
+    prof = c[iz, iy, :]
+    # rank 1: every x line is a multiple of the line through the maximum
+    assert np.allclose(c[iz + 1, iy - 1, :] / c[iz + 1, iy - 1, ix], prof / prof[ix], rtol=1e-12)
+    # within half a box of the centre the log is an exact parabola
+    j = (ix + np.arange(-n // 4, n // 4 + 1)) % n
+    d2 = np.diff(np.log(prof[j]), 2)
+    assert np.allclose(d2, -(h**2) / sigma**2, rtol=1e-9)
+
+
+def test_tracer_ic_sum_integral():
+    """16 Gaussians of width 0.3: integral 16 (2 pi)^(3/2) sigma^3 and a maximum
+    of at least one (each bump peaks at 1 at its centre; the grid misses the
+    centres by at most half a cell)."""
+    g = grids.ThreeDGrid(40, 2 * np.pi, x0=0, y0=0, z0=0)
+    sigma = 0.3
+    c = ics.tracer_ic(g, seed=5, ngauss=16, sigma=sigma)
+    h = 2 * np.pi / 40
+    integral = c.sum() * h**3
+    assert abs(integral - 16 * (2 * np.pi) ** 1.5 * sigma**3) < 1e-10 * integral
+    assert c.max() > np.exp(-3 * (h / 2) ** 2 / (2 * sigma**2)) - 1e-12
+    assert c.min() >= 0

@@ -70,6 +70,32 @@ def test_output_host_problem(tmp_path):
     assert list(d["diagnostics"]["mydiagnostic"]["data"]) == [1.0, 1.0, 1.0]
 
 
+def test_background_writes_interleaved_with_synchronous(tmp_path):
+    """Background snapshots interleaved with saveproblem, savediagnostic and
+    synchronous saveoutput on the same path: one valid archive holding every
+    member; the writer is flushed before each synchronous write."""
+    o = _out()
+    clock = types.SimpleNamespace(dt=0.1, t=0.0, step=0)
+    eqn = types.SimpleNamespace(L=-np.arange(5.0), dims=(5,), T=np.complex128)
+    prob = types.SimpleNamespace(clock=clock, eqn=eqn, grid=object(), params=None, c=np.arange(4096.0))
+    out = o.Output(prob, str(tmp_path / "mixed"), ("c", lambda p: p.c * (1 + p.clock.step)))
+    o.saveproblem(out)
+    for k in range(1, 25):
+        clock.step, clock.t = k, 0.1 * k
+        o.saveoutput(out, background=(k % 3 != 0))
+        if k % 4 == 0:
+            diag = types.SimpleNamespace(i=2, steps=np.array([0, k]), t=np.array([0.0, clock.t]), data=[1.0, 2.0])
+            o.savediagnostic(diag, f"d{k}", out.path)
+    out.close()
+    d = o.load(out.path)
+    assert sorted(int(s) for s in d["snapshots"]["c"]) == list(range(1, 25))
+    for k in range(1, 25):
+        assert np.array_equal(d["snapshots"]["c"][str(k)], np.arange(4096.0) * (1 + k))
+        assert d["snapshots"]["t"][str(k)] == 0.1 * k
+    assert sorted(d["diagnostics"]) == [f"d{k}" for k in sorted(range(4, 25, 4), key=str)]
+    assert list(d["eqn"]["dims"]) == [5]
+
+
 @pytest.mark.gpu
 def test_saveproblem_saveoutput_gpu(ff, tmp_path):
     """test/test_output.jl:56-77 on the diffusion testbed with array kappa."""

@@ -213,6 +213,21 @@ __device__ __forceinline__ void twr_fill_roots(TwRegs<N, P, R, NREG>& t, const c
   }
 }
 
+// Barrier of a team: the whole workgroup, or (WAVE) one wavefront that is the
+// whole team.  A wavefront's LDS operations execute in issue order, so the
+// wavefront-scope fences and the wave barrier only stop the compiler from
+// moving LDS reads and writes of different lanes across the exchange.
+template <bool WAVE>
+__device__ __forceinline__ void team_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One pass in two halves: tp_load gathers a thread's butterfly inputs (from
 // registers on the first pass, else from LDS), tp_exec applies twiddles and
 // butterflies and writes the outputs (to registers on the last pass, else to
@@ -266,30 +281,32 @@ __device__ __forceinline__ void tp_exec(cplx<R> (&v)[P], cplx<R> (&x)[P], cplx<R
   }
 }
 
-template <int N, int P, int RK, int NS, int TWO, int TRO, bool REG, bool FIRST, bool LAST, int DIR, typename R>
+template <int N, int P, int RK, int NS, int TWO, int TRO, bool REG, bool FIRST, bool LAST, int DIR, typename R,
+          bool WAVE = false>
 __device__ __forceinline__ void team_pass(cplx<R> (&v)[P], cplx<R>* __restrict__ lds, int tid,
                                           const cplx<R>* __restrict__ tw, const cplx<R>* twr) {
   cplx<R> x[P];
   tp_load<N, P, RK, FIRST, R>(v, x, lds, tid);
-  if constexpr (!FIRST) __syncthreads();
+  if constexpr (!FIRST) team_sync<WAVE>();
   tp_exec<N, P, RK, NS, TWO, TRO, REG, LAST, DIR, R>(v, x, lds, tid, tw, twr);
-  if constexpr (!LAST) __syncthreads();
+  if constexpr (!LAST) team_sync<WAVE>();
 }
 
-template <int N, int P, int RK, int NS, int TWO, int TRO, bool REG, bool FIRST, bool LAST, int DIR, typename R>
+template <int N, int P, int RK, int NS, int TWO, int TRO, bool REG, bool FIRST, bool LAST, int DIR, typename R,
+          bool WAVE = false>
 __device__ __forceinline__ void team_pass2(cplx<R> (&v1)[P], cplx<R> (&v2)[P], cplx<R>* __restrict__ lds1,
                                            cplx<R>* __restrict__ lds2, int tid, const cplx<R>* __restrict__ tw,
                                            const cplx<R>* twr) {
   cplx<R> x1[P], x2[P];
   tp_load<N, P, RK, FIRST, R>(v1, x1, lds1, tid);
   tp_load<N, P, RK, FIRST, R>(v2, x2, lds2, tid);
-  if constexpr (!FIRST) __syncthreads();
+  if constexpr (!FIRST) team_sync<WAVE>();
   tp_exec<N, P, RK, NS, TWO, TRO, REG, LAST, DIR, R>(v1, x1, lds1, tid, tw, twr);
   tp_exec<N, P, RK, NS, TWO, TRO, REG, LAST, DIR, R>(v2, x2, lds2, tid, tw, twr);
-  if constexpr (!LAST) __syncthreads();
+  if constexpr (!LAST) team_sync<WAVE>();
 }
 
-template <int N, int P, int S, int DIR, int NREG, typename R>
+template <int N, int P, int S, int DIR, int NREG, typename R, bool WAVE = false>
 __device__ __forceinline__ void team_passes(cplx<R> (&v)[P], cplx<R>* lds, int tid, const cplx<R>* tw,
                                             const cplx<R>* twr) {
   using TS = TeamSched<N, P>;
@@ -297,12 +314,12 @@ __device__ __forceinline__ void team_passes(cplx<R> (&v)[P], cplx<R>* lds, int t
     constexpr int RK = S < TS::NFULL ? P : TS::REM;
     constexpr int NS = cpow(P, S);
     team_pass<N, P, RK, NS, tw_off<N, P, S>(), twr_off<N, P, S>(), (S <= NREG), S == 0, S == TS::NPASS - 1, DIR,
-              R>(v, lds, tid, tw, twr);
-    team_passes<N, P, S + 1, DIR, NREG, R>(v, lds, tid, tw, twr);
+              R, WAVE>(v, lds, tid, tw, twr);
+    team_passes<N, P, S + 1, DIR, NREG, R, WAVE>(v, lds, tid, tw, twr);
   }
 }
 
-template <int N, int P, int S, int DIR, int NREG, typename R>
+template <int N, int P, int S, int DIR, int NREG, typename R, bool WAVE = false>
 __device__ __forceinline__ void team_passes2(cplx<R> (&v1)[P], cplx<R> (&v2)[P], cplx<R>* lds1, cplx<R>* lds2,
                                              int tid, const cplx<R>* tw, const cplx<R>* twr) {
   using TS = TeamSched<N, P>;
@@ -310,8 +327,8 @@ __device__ __forceinline__ void team_passes2(cplx<R> (&v1)[P], cplx<R> (&v2)[P],
     constexpr int RK = S < TS::NFULL ? P : TS::REM;
     constexpr int NS = cpow(P, S);
     team_pass2<N, P, RK, NS, tw_off<N, P, S>(), twr_off<N, P, S>(), (S <= NREG), S == 0, S == TS::NPASS - 1, DIR,
-               R>(v1, v2, lds1, lds2, tid, tw, twr);
-    team_passes2<N, P, S + 1, DIR, NREG, R>(v1, v2, lds1, lds2, tid, tw, twr);
+               R, WAVE>(v1, v2, lds1, lds2, tid, tw, twr);
+    team_passes2<N, P, S + 1, DIR, NREG, R, WAVE>(v1, v2, lds1, lds2, tid, tw, twr);
   }
 }
 
@@ -349,6 +366,46 @@ __device__ __forceinline__ void team_fft(cplx<R> (&v)[P], cplx<R>* lds, int tid,
                                          const TwRegs<N, P, R, NREG>& t) {
   static_assert(TeamSched<N, P>::NPASS >= 2, "team_fft needs at least two passes");
   team_passes<N, P, 0, DIR, NREG, R>(v, lds, tid, tw, t.w);
+}
+
+// One-wavefront teams (T = N/P = 64 lanes, the team is the wavefront): the
+// same transforms with wavefront-level exchange barriers, so teams never
+// wait for each other.  Every twiddle comes from registers (t.NR covers all
+// passes that have factors), so no LDS table is needed.
+template <int N, int P, int DIR, typename R, int NREG>
+__device__ __forceinline__ void wave_fft(cplx<R> (&v)[P], cplx<R>* lds, int lane, const TwRegs<N, P, R, NREG>& t) {
+  static_assert(N / P == 64, "a wavefront team has 64 lanes");
+  static_assert(NREG >= TeamSched<N, P>::NPASS - 1, "wave_fft keeps every twiddle in registers");
+  team_passes<N, P, 0, DIR, NREG, R, true>(v, lds, lane, nullptr, t.w);
+}
+template <int N, int P, int DIR, typename R, int NREG>
+__device__ __forceinline__ void wave_fft2(cplx<R> (&v1)[P], cplx<R> (&v2)[P], cplx<R>* lds1, cplx<R>* lds2, int lane,
+                                          const TwRegs<N, P, R, NREG>& t) {
+  static_assert(N / P == 64, "a wavefront team has 64 lanes");
+  static_assert(NREG >= TeamSched<N, P>::NPASS - 1, "wave_fft keeps every twiddle in registers");
+  team_passes2<N, P, 0, DIR, NREG, R, true>(v1, v2, lds1, lds2, lane, nullptr, t.w);
+}
+
+// Registers of a half-length transform (M = N/2 points) filled from the N
+// roots of unity exp(-2 pi i k/N): the M-point factor of index e is root 2e.
+template <int M, int P, int S, typename R, int NREG>
+__device__ __forceinline__ void twr_fill_roots_half(TwRegs<M, P, R, NREG>& t, const cplx<R>* roots, int tid) {
+  using TS = TeamSched<M, P>;
+  if constexpr (S < TS::NPASS && S <= NREG) {
+    constexpr int RK = S < TS::NFULL ? P : TS::REM;
+    constexpr int NS = cpow(P, S);
+    constexpr int NB = P / RK;
+    constexpr int T = M / P;
+    constexpr int STEP = M / (NS * RK);
+    constexpr int RO = twr_off<M, P, S>();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int k = (tid + b * T) & (NS - 1);
+#pragma unroll
+      for (int r = 1; r < RK; ++r) t.w[RO + b * (RK - 1) + r - 1] = roots[2 * r * k * STEP];
+    }
+    twr_fill_roots_half<M, P, S + 1, R, NREG>(t, roots, tid);
+  }
 }
 
 }  // namespace ffhip

@@ -118,7 +118,15 @@ struct VortState {
   int64_t ck_nsteps = -1, ck_ncheck = 0;
   int64_t ck_id = 0;     // identifies that run (0: none); each forward_ckpt call takes a new one
   bool fs_last = false;  // FS holds the last segment's fields from the forward run
+  // kVortChunk captured steps of the forward run, replayed as one graph launch;
+  // the graph holds the stepper's packed-coefficient pointer it was captured with
+  hipStream_t cap = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* graph_packed = nullptr;
+  bool graph_filter = false;
 };
+
+constexpr int kVortChunk = 10;  // ETDRK4 steps per captured graph (80 kernel nodes)
 
 __device__ __forceinline__ int vfreq(int i, int n) { return i < n / 2 ? i : i - n; }
 

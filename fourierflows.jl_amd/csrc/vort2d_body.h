@@ -391,6 +391,132 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
   }
 }
 
+// The row pass of one x-row y by one wavefront (FP32), with half-length
+// transforms: every transform is an M = NX/2-point complex FFT of Q = M/64
+// points per lane, exchanged through the wavefront's own LDS buffers with
+// wavefront-level barriers only, so the rows of a workgroup never wait for
+// each other.  Lane l holds the sample pairs x = 2n, 2n+1 for n = l + 64 q.
+//   zeta:   the real inverse as one M-point transform of
+//           Z[k] = X[k] + conj(X[M-k]) + i (X[k] - conj(X[M-k])) w^k,
+//           giving zeta(2n) + i zeta(2n+1)              (w = exp(2 pi i/NX))
+//   u + iv: decimation in frequency, A[k] = S[k] + S[k+M] and
+//           B[k] = (S[k] - S[k+M]) w^k, two M-point inverses giving
+//           (u + iv)(2n) and (u + iv)(2n+1)
+//   forward: decimation in time on the products' even and odd samples, two
+//           M-point transforms E, O and W[k] = E + w^-k O, W[k+M] = E - w^-k O
+//   split:  P[k], Q[k] from W[k] and conj(W[NX-k]); W[NX-k] sits in lane
+//           (64 - l) mod 64, fetched with ds_bpermute (__shfl)
+// Five M-point transforms per row instead of three NX-point ones.  lds1 and
+// lds2 are this wavefront's two buffers of TeamSched<M, Q>::LDS slots; every
+// twiddle is in registers (filled from the NX roots of unity in tw).
+template <int NX, int Q>
+__device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs& a, const int y, const int lane,
+                                           cplx<float>* lds1, cplx<float>* lds2) {
+  using R = float;
+  constexpr int M = NX / 2;
+  static_assert(M / Q == 64, "one wavefront per row");
+  constexpr int CB = kVCB;
+  const uint32_t ny = (uint32_t)g.ny;
+  auto iaddr = [&](int k) -> uint32_t { return ((uint32_t)(k / CB) * ny + (uint32_t)y) * CB + (k % CB); };
+  const cplx<R>* roots = reinterpret_cast<const cplx<R>*>(a.tw);  // exp(-2 pi i k/NX)
+  const cplx<R>* Zh = reinterpret_cast<const cplx<R>*>(a.I0);
+  const cplx<R>* Uh = reinterpret_cast<const cplx<R>*>(a.I1);
+  const cplx<R>* Vh = reinterpret_cast<const cplx<R>*>(a.I2);
+  const int KX = g.KX;
+  TwRegs<M, Q, R> twr;
+  twr_fill_roots_half<M, Q, 1, R, TwRegs<M, Q, R>::NR>(twr, roots, lane);
+
+  // zeta (2n) + i zeta (2n + 1).  Branch-free loads: out-of-band modes read
+  // column 0 and are discarded; c2r ignores Im at kx = 0 and the Nyquist
+  // column (k = M) is never kept.
+  cplx<R> zr[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int k = lane + 64 * q, m = M - k;
+    const bool okk = k < KX, okm = m < KX;
+    const cplx<R> Xk = at32(Zh, iaddr(okk ? k : 0)), Xm = at32(Zh, iaddr(okm ? m : 0));
+    const cplx<R> om = roots[k];
+    const R xkx = okk ? Xk.x : R(0), xky = (okk && k != 0) ? Xk.y : R(0);
+    const R xmx = okm ? Xm.x : R(0), xmy = okm ? Xm.y : R(0);
+    const R dx = xkx - xmx, dy = xky + xmy;                           // X[k] - conj(X[M-k])
+    const R aox = dx * om.x + dy * om.y, aoy = dy * om.x - dx * om.y;  // times w^k = conj(root k)
+    zr[q] = mkc<R>(xkx + xmx - aoy, xky - xmy + aox);                 // Ae + i Ao
+  }
+  wave_fft<M, Q, 1, R>(zr, lds1, lane, twr);
+
+  // u + i v at the even (xa) and odd (xb) samples
+  cplx<R> xa[Q], xb[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int k = lane + 64 * q, m = M - k;
+    const bool okk = k < KX, okm = m < KX;
+    const uint32_t ik = iaddr(okk ? k : 0), im = iaddr(okm ? m : 0);
+    const cplx<R> U = at32(Uh, ik), V = at32(Vh, ik), Um = at32(Uh, im), Vm = at32(Vh, im);
+    const cplx<R> om = roots[k];
+    const R uy = k == 0 ? R(0) : U.y, vy = k == 0 ? R(0) : V.y;
+    const R slx = okk ? U.x - vy : R(0), sly = okk ? uy + V.x : R(0);       // S[k] = U + i V
+    const R shx = okm ? Um.x + Vm.y : R(0), shy = okm ? Vm.x - Um.y : R(0);  // S[k+M] = conj(U) + i conj(V) at M-k
+    xa[q] = mkc<R>(slx + shx, sly + shy);
+    const R dx = slx - shx, dy = sly - shy;
+    xb[q] = mkc<R>(dx * om.x + dy * om.y, dy * om.x - dx * om.y);
+  }
+  wave_fft2<M, Q, 1, R>(xa, xb, lds1, lds2, lane, twr);
+
+  // products u zeta + i v zeta with the 1/(nx ny)^2 normalisation
+  const R s2 = (R)g.s2;
+  if (a.F) {  // keep u + i v and zeta (normalised) for the adjoint sweep
+    const R s1 = (R)(1.0 / ((double)g.nx * (double)g.ny));
+    cplx<R>* Fz = reinterpret_cast<cplx<R>*>(a.F) + (int64_t)y * NX;
+    R* Fr = reinterpret_cast<R*>(reinterpret_cast<cplx<R>*>(a.F) + (int64_t)g.nx * g.ny) + (int64_t)y * NX;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int n = lane + 64 * q;
+      reinterpret_cast<float4*>(Fz)[n] = make_float4(xa[q].x * s1, xa[q].y * s1, xb[q].x * s1, xb[q].y * s1);
+      reinterpret_cast<float2*>(Fr)[n] = make_float2(zr[q].x * s1, zr[q].y * s1);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const R ze = zr[q].x * s2, zo = zr[q].y * s2;
+    xa[q] = mkc<R>(xa[q].x * ze, xa[q].y * ze);
+    xb[q] = mkc<R>(xb[q].x * zo, xb[q].y * zo);
+  }
+  wave_fft2<M, Q, -1, R>(xa, xb, lds1, lds2, lane, twr);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {  // W[k] in xa, W[k + M] in xb
+    const cplx<R> t = cmul<R>(roots[lane + 64 * q], xb[q]);
+    const cplx<R> e = xa[q];
+    xa[q] = mkc<R>(e.x + t.x, e.y + t.y);
+    xb[q] = mkc<R>(e.x - t.x, e.y - t.y);
+  }
+
+  // split W into the half spectra of its real and imaginary parts.  For
+  // k = l + 64 q >= 1, NX - k = M + (M - k) and M - k = (64 - l) + 64 (Q-1-q)
+  // (lane l >= 1) or 64 (Q - q) (lane 0).
+  cplx<R>* O0 = reinterpret_cast<cplx<R>*>(a.O0);
+  cplx<R>* O1 = reinterpret_cast<cplx<R>*>(a.O1);
+  const int partner = (64 - lane) & 63;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    if (64 * q >= g.KXp) break;  // uniform over the wavefront: the shuffles below run in every lane
+    cplx<R> mir;
+    mir.x = __shfl(xb[Q - 1 - q].x, partner, 64);
+    mir.y = __shfl(xb[Q - 1 - q].y, partner, 64);
+    if (lane == 0) mir = q == 0 ? xa[0] : xb[q == 0 ? 0 : Q - q];
+    const int k = lane + 64 * q;
+    if (k < g.KXp) {
+      cplx<R> Pk = mkc<R>(0, 0), Qk = mkc<R>(0, 0);
+      if (k < KX) {
+        const cplx<R> wk = xa[q], wc = cconj<R>(mir);
+        Pk = mkc<R>(R(0.5) * (wk.x + wc.x), R(0.5) * (wk.y + wc.y));
+        Qk = mkc<R>(R(0.5) * (wk.y - wc.y), -R(0.5) * (wk.x - wc.x));  // (wk - wc) / (2i)
+      }
+      at32(O0, iaddr(k)) = Pk;
+      at32(O1, iaddr(k)) = Qk;
+    }
+  }
+}
+
 // The row pass of two x-rows (y0, y0 + 1) by three teams of T = NX/P threads
 // (T a multiple of 64: wave-uniform team branches).  Team 0 inverts u + i v
 // of row y0, team 1 of row y0 + 1, team 2 zeta(y0) + i zeta(y0 + 1) -- three

@@ -128,6 +128,27 @@ __global__ void __launch_bounds__(kRRK*(NX / P), sizeof(R) == 4 ? FF_ROW_WAVES :
   }
 }
 
+// FP32 rows by wavefronts with half-length transforms (vrowh_body): one
+// 4-row group (whole 128 B lines of the blocked arrays) per workgroup of four
+// wavefronts, one row each, two LDS buffers per wavefront.
+#ifndef FF_ROWH
+#define FF_ROWH 1  // FP32 1024 and 2048 row passes use the wavefront kernel
+#endif
+template <int NX>
+constexpr bool rowh_on() {
+  return FF_ROWH && (NX == 1024 || NX == 2048);
+}
+template <int NX>
+__global__ void __launch_bounds__(256) k_vrowh(VortGeom g, VortRowArgs a) {
+  extern __shared__ unsigned char vsm[];
+  constexpr int M = NX / 2, Q = M / 64;
+  constexpr int SL = TeamSched<M, Q>::LDS;
+  const int grp = blockIdx.x;  // grid = ny / 4
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  cplx<float>* base = reinterpret_cast<cplx<float>*>(vsm) + 2 * w * SL;
+  vrowh_body<NX, Q>(g, a, grp * 4 + w, lane, base, base + SL);
+}
+
 // ----------------------------------------------------------------- host side
 static bool pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
@@ -171,6 +192,15 @@ static bool use3(const char* var, bool f32) {
   const char* e = getenv(var);
   const int mode = e ? atoi(e) : 1;
   return f32 ? mode >= 2 : mode >= 1;
+}
+// FFHIP_ROWH=0 selects the two-row team kernel for FP32 rows where the
+// wavefront kernel applies (default on)
+static bool rowh_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("FFHIP_ROWH");
+    return !e || atoi(e) != 0;
+  }();
+  return on;
 }
 static bool col3_on(bool f32) {
   static const bool on32 = use3("FFHIP_COL3", true), on64 = use3("FFHIP_COL3", false);
@@ -218,8 +248,28 @@ static int launch_row3(const VortGeom& g, const VortRowArgs& a, hipStream_t st) 
   return FFHIP_OK;
 }
 
+template <int N>
+static int launch_rowh(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
+  constexpr int M = N / 2, Q = M / 64;
+  const size_t sh = (size_t)8 * TeamSched<M, Q>::LDS * sizeof(cplx<float>);
+  auto kern = k_vrowh<N>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    FF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)sh));
+    attr_set = true;
+  }
+  FF_CHECK(g.ny % 4 == 0, FFHIP_ERR_UNSUPPORTED, "wavefront row kernel: ny = %d", g.ny);
+  prof_launch("vort_row", row_bytes<float>(g), kern, dim3(g.ny / 4), dim3(256), sh, st, g, a);
+  FF_LAUNCHED("k_vrowh");
+  return FFHIP_OK;
+}
+
 template <typename R, int N>
 static int launch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
+  if constexpr (sizeof(R) == 4 && rowh_on<N>()) {
+    if (rowh_enabled()) return launch_rowh<N>(g, a, st);
+  }
   constexpr int P = vrow_pts<R, N>();
   const size_t sh = row_lds<R, N>();
   auto kern = k_vrow<R, N, P>;
@@ -280,8 +330,16 @@ static int dispatch_stage(int stage, const VortGeom& g, const VortColArgs& a, hi
   }
 }
 
+static void drop_graph(VortState* s) {
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  s->exec = nullptr;
+  s->graph_packed = nullptr;
+}
+
 static void vstate_free(VortState* s) {
   if (!s) return;
+  drop_graph(s);
+  if (s->cap) (void)hipStreamDestroy(s->cap);
   dev_free(s->U);
   dev_free(s->N1);
   dev_free(s->N2);
@@ -347,6 +405,15 @@ int fused_create(ffhip_op* op) {
 void fused_destroy(ffhip_op* op) {
   vstate_free(reinterpret_cast<VortState*>(op->fused));
   op->fused = nullptr;
+}
+
+// FFHIP_VORT_GRAPHS=0 keeps every step eager (graph replays are the default)
+static bool vort_graphs_on() {
+  static const bool on = [] {
+    const char* e = getenv("FFHIP_VORT_GRAPHS");
+    return !e || atoi(e) != 0;
+  }();
+  return on;
 }
 
 // packed coefficient arrays (blocked kept band), cached on the stepper
@@ -415,13 +482,50 @@ static int vrun(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, hipSt
   ra.tw = s->twx;
   ra.F = nullptr;
   FF_TRY(dispatch_stage<R>(0, g, ca, st));
-  for (int64_t k = 0; k < nsteps; ++k) {
+  // one step: 4 x (row pass, column stage); the last step of the run skips
+  // its final inverse y-FFTs (nothing reads them)
+  auto step = [&](hipStream_t s_, bool last) -> int {
     for (int stage = 1; stage <= 4; ++stage) {
-      FF_TRY(dispatch_row<R>(g, ra, st));
-      ca.do_inverse = (stage < 4 || k + 1 < nsteps) ? 1 : 0;
-      FF_TRY(dispatch_stage<R>(stage, g, ca, st));
+      FF_TRY(dispatch_row<R>(g, ra, s_));
+      ca.do_inverse = (stage < 4 || !last) ? 1 : 0;
+      FF_TRY(dispatch_stage<R>(stage, g, ca, s_));
     }
+    return FFHIP_OK;
+  };
+  int64_t left = nsteps;
+  // Whole chunks of kVortChunk steps replay a captured graph (no launch gaps
+  // between the 80 kernels).  The first step runs eagerly before any capture
+  // (so every kernel's LDS attribute is set outside it), the last eagerly
+  // because it differs.  Profiled runs stay eager: they time each launch.
+  const bool graphs = !g_prof_on && nsteps >= kVortChunk + 2 && vort_graphs_on();
+  if (graphs) {
+    FF_TRY(step(st, false));
+    --left;
+    if (!s->exec || s->graph_packed != ts->packed || s->graph_filter != (ts->filter != nullptr)) {
+      drop_graph(s);
+      if (!s->cap) FF_HIP(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
+      FF_HIP(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
+      int rc = FFHIP_OK;
+      for (int k = 0; k < kVortChunk && rc == FFHIP_OK; ++k) rc = step(s->cap, false);
+      hipGraph_t graph = nullptr;
+      const hipError_t ee = hipStreamEndCapture(s->cap, &graph);
+      if (rc != FFHIP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (ee != hipSuccess) return fail(FFHIP_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ee));
+      const hipError_t ei = hipGraphInstantiate(&s->exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        s->exec = nullptr;
+        return fail(FFHIP_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+      }
+      s->graph_packed = ts->packed;
+      s->graph_filter = ts->filter != nullptr;
+    }
+    for (; left > kVortChunk; left -= kVortChunk) FF_HIP(hipGraphLaunch(s->exec, st));
   }
+  for (int64_t k = 0; k < left; ++k) FF_TRY(step(st, k + 1 == left));
   const int64_t total = (int64_t)g.ny * g.nkr;
   hipLaunchKernelGGL((k_vexport<R>), dim3(blocks_for(total, 256, 4096)), dim3(256), 0, st,
                      reinterpret_cast<cplx<R>*>(sol), reinterpret_cast<const cplx<R>*>(s->U),

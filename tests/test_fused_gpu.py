@@ -156,6 +156,29 @@ def test_fused_repeated_calls_equal_one_call(ff):
     assert p1.clock.step == p2.clock.step == 20
 
 
+@pytest.mark.parametrize("T,stepper", [(np.float32, "ETDRK4"), (np.float64, "ETDRK4"),
+                                       (np.float32, "FilteredETDRK4")])
+def test_fused_graph_replays_equal_eager_steps(ff, T, stepper):
+    """Runs of >= 12 steps replay a captured 10-step graph; one-step calls are
+    all eager.  Two 35-step runs (three replays each, with the library's eager
+    first and last steps and the import/export kernels between them) must
+    equal 70 one-step calls bit for bit."""
+    n = 256
+    o = og.TwoDGrid(n, 2 * np.pi, x0=0, y0=0)
+    zh0 = oics.vorticity_ic(o, seed=12).astype(np.complex64 if T == np.float32 else np.complex128)
+    p1 = ff.TwoDVorticityProblem(nx=n, nu=1e-3, dt=0.005, T=T, stepper=stepper)
+    p2 = ff.TwoDVorticityProblem(nx=n, nu=1e-3, dt=0.005, T=T, stepper=stepper)
+    assert p1.eqn.calcN.fused
+    p1.sol.copy_(_dev(zh0))
+    p2.sol.copy_(_dev(zh0))
+    for _ in range(2):
+        ff.stepforward(p1, 35)
+        for _ in range(35):
+            ff.stepforward(p2, 1)
+        assert np.array_equal(_host(p1.sol), _host(p2.sol))
+    assert p1.clock.step == p2.clock.step == 70
+
+
 def test_fused_deterministic(ff):
     n = 256
     o = og.TwoDGrid(n, 2 * np.pi, x0=0, y0=0)

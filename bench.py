@@ -166,6 +166,51 @@ def tracer_timing(ff, torch, n=256, nsteps=300, dt=0.0025, kappa=1e-3):
             "note": "dt=0.01 is outside RK4's stability region for this flow at 256^3 (growth 2.4 per step)"}
 
 
+def fft_plan_timing(ff, torch, n=N, reps=50):
+    """mul!/ldiv! on the grid's rfft plan (src/domains.jl:5, src/diffusion.jl:137-139)
+    at n x n in FP32 and FP64: each transform's HBM bytes are its input plus
+    its output (real n^2 and complex (n/2+1) n arrays), over the event-timed
+    average of `reps` calls; per-kernel averages from the library profiler."""
+    import numpy as np
+
+    from fourierflows_hip import _lib
+
+    out = {}
+    for T, rt, ct in ((32, torch.float32, torch.complex64), (64, torch.float64, torch.complex128)):
+        plan = ff.Plan((n, n), np.float32 if T == 32 else np.float64, "rfft")
+        x = torch.randn(n, n, dtype=rt, device="cuda")
+        X = torch.zeros(n, n // 2 + 1, dtype=ct, device="cuda")
+        y = torch.zeros_like(x)
+        byts = x.numel() * x.element_size() + X.numel() * X.element_size()
+        rec = {}
+        for name, fn in (("mul", lambda: plan.forward(X, x)), ("ldiv", lambda: plan.inverse(y, X))):
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1) / 1000.0 / reps
+            _lib.profile_start()
+            for _ in range(5):
+                fn()
+            prof = _lib.profile_stop()
+            kern = {k["name"]: {"count": k["count"] // 5, "avg_us": round(1e3 * k["total_ms"] / k["count"], 3),
+                                "GBps": round(k["bytes"] / k["count"] / (k["total_ms"] / k["count"] * 1e-3) / 1e9, 1)}
+                    for k in prof["kernels"]}
+            gbps = byts / t / 1e9
+            rec[name] = {"us": round(t * 1e6, 3), "io_bytes": byts, "GBps": round(gbps, 1),
+                         "frac": round(gbps / HBM_PEAK_GBS, 4), "kernels": kern}
+        err = float(torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x))
+        rec["roundtrip_rel_l2"] = err
+        out[f"fp{T}"] = rec
+        del plan
+    return {"workload": f"rfft plan {n}x{n}: mul! (r2c) and ldiv! (c2r), device arrays", **out}
+
+
 def algorithmic_bytes(kernel, n, cplx_bytes):
     """Algorithmic bytes of one launch of a fused 2D kernel on an n x n grid,
     from the per-step model of the unfused algorithm: S = (n/2+1) n complex
@@ -236,6 +281,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost measurement")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the mixed-grid ensemble measurement")
     ap.add_argument("--no-tracer", action="store_true", help="skip the 256^3 tracer measurement")
+    ap.add_argument("--no-fft", action="store_true", help="skip the rfft plan (mul!/ldiv!) measurement")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE)")
@@ -316,6 +362,12 @@ def main():
     if not a.no_ensemble:
         ens = ensemble_timing(ff, torch)
         print(f"# ensemble: {ens}", file=sys.stderr, flush=True)
+    fft = None
+    if not a.no_fft:
+        fft = fft_plan_timing(ff, torch)
+        print(f"# fft plan: fp32 mul {fft['fp32']['mul']['GBps']} GB/s, ldiv {fft['fp32']['ldiv']['GBps']} GB/s; "
+              f"fp64 mul {fft['fp64']['mul']['GBps']} GB/s, ldiv {fft['fp64']['ldiv']['GBps']} GB/s",
+              file=sys.stderr, flush=True)
     trc = None
     if not a.no_tracer:
         trc = tracer_timing(ff, torch)
@@ -338,6 +390,7 @@ def main():
         "adjoint": adj,
         "ensemble": ens,
         "tracer": trc,
+        "fft_plan": fft,
         "step_kept_band_GB": round(step_bytes / 1e9, 4),
         "step_kept_band_GBps": round(step_bytes / (t32 / a.steps) / 1e9, 1),
         "step_algorithmic_GB": round(70 * (N // 2 + 1) * N * 8 / 1e9, 4),

@@ -18,6 +18,7 @@
 
 #include "fft_device.h"
 #include "plan.h"
+#include "prof.h"
 
 namespace ffhip {
 
@@ -338,8 +339,8 @@ static int global_axis(const cplx<R>* src, cplx<R>* dst, Layout ls, Layout ld, i
   const cplx<R>* cur = src;
   if (nst == 1 && (const void*)src == (const void*)dst) {
     // a single pass cannot run in place: stage the input in w1 first
-    hipLaunchKernelGGL((k_gather<R>), dim3(blocks_for(ntrans * n, kThreads, 1 << 20)), dim3(kThreads), 0, st, src,
-                       w1, ls, lw, ntrans, n);
+    prof_launch("fft_gather", 2.0 * ntrans * n * sizeof(cplx<R>), k_gather<R>,
+                dim3(blocks_for(ntrans * n, kThreads, 1 << 20)), dim3(kThreads), 0, st, src, w1, ls, lw, ntrans, n);
     FF_LAUNCHED("k_gather");
     cur = w1;
     ls = lw;
@@ -356,13 +357,14 @@ static int global_axis(const cplx<R>* src, cplx<R>* dst, Layout ls, Layout ld, i
     const int64_t total = ntrans * (n / p);
     const unsigned blocks = blocks_for(total, kThreads, 1 << 20);
     const bool tfast = lc.elem != 1;
-#define FF_GPASS(RAD)                                                                                  \
-  if (dir < 0)                                                                                         \
-    hipLaunchKernelGGL((k_pass_global<RAD, -1, R>), dim3(blocks), dim3(kThreads), 0, st, cur, out, lc, lo, \
-                       ntrans, n, p, Ns, tw, sc, tfast);                                               \
-  else                                                                                                 \
-    hipLaunchKernelGGL((k_pass_global<RAD, 1, R>), dim3(blocks), dim3(kThreads), 0, st, cur, out, lc, lo,  \
-                       ntrans, n, p, Ns, tw, sc, tfast);
+    const double pbytes = 2.0 * ntrans * n * sizeof(cplx<R>);
+#define FF_GPASS(RAD)                                                                                   \
+  if (dir < 0)                                                                                          \
+    prof_launch("fft_pass_global", pbytes, k_pass_global<RAD, -1, R>, dim3(blocks), dim3(kThreads), 0, st, \
+                cur, out, lc, lo, ntrans, n, p, Ns, tw, sc, tfast);                                     \
+  else                                                                                                  \
+    prof_launch("fft_pass_global", pbytes, k_pass_global<RAD, 1, R>, dim3(blocks), dim3(kThreads), 0, st,  \
+                cur, out, lc, lo, ntrans, n, p, Ns, tw, sc, tfast);
     switch (p) {
       case 2: FF_GPASS(2); break;
       case 3: FF_GPASS(3); break;
@@ -398,12 +400,13 @@ static int run_rows_c2c(const cplx<R>* in, cplx<R>* out, int64_t nrows, const Ax
     size_t sh = 2 * (size_t)nb * n * sizeof(cplx<R>);
     unsigned blocks = (unsigned)((nrows + nb - 1) / nb);
     const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(ax.tw);
+    const double bytes = 2.0 * nrows * n * sizeof(cplx<R>);
     if (dir < 0)
-      hipLaunchKernelGGL((k_rows_c2c<-1, R>), dim3(blocks), dim3(kThreads), sh, st, in, out, to_radices(ax), tw,
-                         nrows, nb, scale);
+      prof_launch("fft_rows_c2c", bytes, k_rows_c2c<-1, R>, dim3(blocks), dim3(kThreads), sh, st, in, out,
+                  to_radices(ax), tw, nrows, nb, scale);
     else
-      hipLaunchKernelGGL((k_rows_c2c<1, R>), dim3(blocks), dim3(kThreads), sh, st, in, out, to_radices(ax), tw,
-                         nrows, nb, scale);
+      prof_launch("fft_rows_c2c", bytes, k_rows_c2c<1, R>, dim3(blocks), dim3(kThreads), sh, st, in, out,
+                  to_radices(ax), tw, nrows, nb, scale);
     FF_LAUNCHED("k_rows_c2c");
     return FFHIP_OK;
   }
@@ -421,12 +424,13 @@ static int run_cols_c2c(const cplx<R>* in, cplx<R>* out, int64_t ncols, int64_t 
     size_t sh = 2 * (size_t)C * n * sizeof(cplx<R>);
     dim3 grid((unsigned)((ncols + C - 1) / C), (unsigned)nslabs);
     const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(ax.tw);
+    const double bytes = 2.0 * ncols * nslabs * n * sizeof(cplx<R>);
     if (dir < 0)
-      hipLaunchKernelGGL((k_cols_c2c<-1, R>), grid, dim3(kThreads), sh, st, in, out, to_radices(ax), tw, ncols,
-                         stride, slab_dist, C, scale);
+      prof_launch("fft_cols_c2c", bytes, k_cols_c2c<-1, R>, grid, dim3(kThreads), sh, st, in, out, to_radices(ax),
+                  tw, ncols, stride, slab_dist, C, scale);
     else
-      hipLaunchKernelGGL((k_cols_c2c<1, R>), grid, dim3(kThreads), sh, st, in, out, to_radices(ax), tw, ncols,
-                         stride, slab_dist, C, scale);
+      prof_launch("fft_cols_c2c", bytes, k_cols_c2c<1, R>, grid, dim3(kThreads), sh, st, in, out, to_radices(ax),
+                  tw, ncols, stride, slab_dist, C, scale);
     FF_LAUNCHED("k_cols_c2c");
     return FFHIP_OK;
   }
@@ -445,8 +449,8 @@ static int run_rows_r2c(const R* in, cplx<R>* out, int64_t nrows, ffhip_plan* p,
     const int nb = ax.nb;
     size_t sh = 2 * (size_t)nb * m * sizeof(cplx<R>);
     unsigned blocks = (unsigned)((nrows + nb - 1) / nb);
-    hipLaunchKernelGGL((k_rows_r2c<R>), dim3(blocks), dim3(kThreads), sh, st, in, out, to_radices(ax), tw, rtw,
-                       nrows, nb);
+    prof_launch("fft_rows_r2c", (double)nrows * (2.0 * m * sizeof(R) + (m + 1) * sizeof(cplx<R>)), k_rows_r2c<R>,
+                dim3(blocks), dim3(kThreads), sh, st, in, out, to_radices(ax), tw, rtw, nrows, nb);
     FF_LAUNCHED("k_rows_r2c");
     return FFHIP_OK;
   }
@@ -454,8 +458,9 @@ static int run_rows_r2c(const R* in, cplx<R>* out, int64_t nrows, ffhip_plan* p,
   Layout L{1, 0, ax.n, 1};
   FF_TRY(global_axis<R>(reinterpret_cast<const cplx<R>*>(in), Z, L, L, nrows, ax, -1, R(1),
                         reinterpret_cast<cplx<R>*>(p->g0), reinterpret_cast<cplx<R>*>(p->g1), st));
-  hipLaunchKernelGGL((k_r2c_split<R>), dim3(blocks_for(nrows * (m + 1), kThreads, 1 << 20)), dim3(kThreads), 0,
-                     st, Z, out, rtw, nrows, (int64_t)m);
+  prof_launch("fft_r2c_split", (double)nrows * (2 * m + 1) * sizeof(cplx<R>), k_r2c_split<R>,
+              dim3(blocks_for(nrows * (m + 1), kThreads, 1 << 20)), dim3(kThreads), 0, st, Z, out, rtw, nrows,
+              (int64_t)m);
   FF_LAUNCHED("k_r2c_split");
   return FFHIP_OK;
 }
@@ -470,14 +475,14 @@ static int run_rows_c2r(const cplx<R>* in, R* out, int64_t nrows, R scale, ffhip
     const int nb = ax.nb;
     size_t sh = 2 * (size_t)nb * m * sizeof(cplx<R>);
     unsigned blocks = (unsigned)((nrows + nb - 1) / nb);
-    hipLaunchKernelGGL((k_rows_c2r<R>), dim3(blocks), dim3(kThreads), sh, st, in, out, to_radices(ax), tw, rtw,
-                       nrows, nb, scale);
+    prof_launch("fft_rows_c2r", (double)nrows * (2.0 * m * sizeof(R) + (m + 1) * sizeof(cplx<R>)), k_rows_c2r<R>,
+                dim3(blocks), dim3(kThreads), sh, st, in, out, to_radices(ax), tw, rtw, nrows, nb, scale);
     FF_LAUNCHED("k_rows_c2r");
     return FFHIP_OK;
   }
   cplx<R>* Z = reinterpret_cast<cplx<R>*>(p->g2);
-  hipLaunchKernelGGL((k_c2r_merge<R>), dim3(blocks_for(nrows * m, kThreads, 1 << 20)), dim3(kThreads), 0, st, in,
-                     Z, rtw, nrows, (int64_t)m);
+  prof_launch("fft_c2r_merge", (double)nrows * (2 * m + 1) * sizeof(cplx<R>), k_c2r_merge<R>,
+              dim3(blocks_for(nrows * m, kThreads, 1 << 20)), dim3(kThreads), 0, st, in, Z, rtw, nrows, (int64_t)m);
   FF_LAUNCHED("k_c2r_merge");
   Layout L{1, 0, ax.n, 1};
   return global_axis<R>(Z, reinterpret_cast<cplx<R>*>(out), L, L, nrows, ax, 1, scale,

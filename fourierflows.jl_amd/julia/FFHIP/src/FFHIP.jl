@@ -19,7 +19,7 @@ module FFHIP
 
 using LinearAlgebra
 
-const libffhip = get(ENV, "FFHIP_LIB", joinpath(@__DIR__, "..", "lib", "libffhip.so"))
+const libffhip = get(ENV, "FFHIP_LIB", joinpath(@__DIR__, "..", "..", "..", "lib", "libffhip.so"))
 
 const OK = Cint(0)
 const F32 = Cint(32)
@@ -81,6 +81,16 @@ function device_array(::HIP)
         a
     end
 end
+
+Base.similar(a::HIPArray{T,N}) where {T,N} = HIPArray{T,N}(a.dims)
+
+function Base.copyto!(dst::HIPArray{T,N}, src::HIPArray{T,N}; stream=C_NULL) where {T,N}
+    dst.dims == src.dims || throw(DimensionMismatch("copyto!: $(dst.dims) vs $(src.dims)"))
+    check(ccall((:ffhip_memcpy_dtod, libffhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+                dst, src, prod(src.dims) * sizeof(T), stream))
+    return dst
+end
+Base.copy(a::HIPArray) = copyto!(similar(a), a)
 
 function Base.Array(a::HIPArray{T,N}) where {T,N}
     x = Array{T,N}(undef, a.dims)
@@ -232,17 +242,25 @@ etdrk4_adjoint!(op, ts, lam0, lamT, nsteps, ncheck; stream=C_NULL) =
                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Ptr{Cvoid}),
                 op.h, ts.h, lamT, lam0, nsteps, ncheck, stream))
 
-# With ChainRulesCore loaded, the rrule of an n-step solve is:
-#
-#   function ChainRulesCore.rrule(::typeof(solve), op, ts, sol0, nsteps, ncheck)
-#       solT = similar(sol0); etdrk4_forward_ckpt!(op, ts, solT, sol0, nsteps, ncheck)
-#       pullback(dsolT) = (NoTangent(), NoTangent(), NoTangent(),
-#                          (lam = copy(unthunk(dsolT)); etdrk4_adjoint!(op, ts, lam, lam, nsteps, ncheck); lam),
-#                          NoTangent(), NoTangent())
-#       return solT, pullback
-#   end
-#
-# Julia's complex cotangent convention (dL/dRe + i dL/dIm) is the transpose
-# for the real inner product Re sum conj(a) b that libffhip's adjoint uses.
+"Identifier of the checkpointed forward run `op` holds (a new one per `etdrk4_forward_ckpt!`)."
+function checkpoint_id(op::BuiltinOperator)
+    id = Ref{Int64}(0)
+    check(ccall((:ffhip_op_checkpoint_id, libffhip), Cint, (Ptr{Cvoid}, Ref{Int64}), op.h, id))
+    return id[]
+end
+
+"""
+    solve(op, ts, sol0, nsteps, ncheck) -> solT
+
+`nsteps` fused ETDRK4 steps from `sol0` (left unchanged), keeping `ncheck`
+checkpoints so that the reverse sweep can run.  ChainRulesCore's `rrule` for
+it lives in the package extension `FFHIPChainRulesCoreExt` (ext/), loaded
+when ChainRulesCore is.
+"""
+function solve(op::BuiltinOperator, ts::ETDRK4TimeStepper, sol0::HIPArray, nsteps::Integer, ncheck::Integer)
+    solT = similar(sol0)
+    etdrk4_forward_ckpt!(op, ts, solT, sol0, nsteps, ncheck)
+    return solT
+end
 
 end # module

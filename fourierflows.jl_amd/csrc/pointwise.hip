@@ -356,6 +356,45 @@ int parseval_launch(int mode, const void* uh, int prec, int ndim, const int64_t*
   return FFHIP_OK;
 }
 
+// ----------------------------------------------------------------- allocation-free reductions
+// ffhip_parseval_noalloc keeps its partial sums in this fixed array instead of
+// a library allocation: kRedBlocks double-double partials (16 KiB), one copy
+// per GPU that the library runs on, held until the process ends.  In host code
+// the name is a host-side stand-in, not a device address, so it is never
+// passed to a kernel: red_scratch_setup() looks up each device's address once
+// (hipGetSymbolAddress, that device current) and every launch takes that
+// pointer.  Each launch writes partials 0..blocks-1 before the final kernel
+// reads them, so nothing needs clearing between calls; calls that overlap on
+// one GPU (concurrent streams or threads) race on the array.
+static __device__ __align__(256) double2 scratch[kRedBlocks];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double2* g_scratch[kMaxDevices] = {};  // device address of `scratch`, per device
+
+int red_scratch_setup(int device) {
+  FF_CHECK(device >= 0 && device < kMaxDevices, FFHIP_ERR_ARG, "device %d beyond the scratch table (%d)", device,
+           kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return FFHIP_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(FFHIP_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device, hipGetErrorString(e));
+  g_scratch[device] = reinterpret_cast<double2*>(p);
+  return FFHIP_OK;
+}
+
+// (acc.x + acc.y) * norm of the partials, written to device memory
+__global__ void __launch_bounds__(kRedThreads) k_dd_final_scaled(const double2* __restrict__ partial, int n,
+                                                                 double norm, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  double2 acc = make_double2(0, 0);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc = dd_add(acc, partial[i]);
+  acc = block_dd_sum(acc);
+  if (threadIdx.x == 0) out[0] = (acc.x + acc.y) * norm;
+}
+
 // ----------------------------------------------------------------- steppers: create/destroy
 static int upload_f64(void** d, const double* h, int64_t count) {
   FF_TRY(dev_alloc(d, (size_t)count * sizeof(double)));
@@ -490,6 +529,42 @@ int ffhip_parsevalsum(const void* uh, int prec, int ndim, const int64_t* dims, i
                       double* out, void* stream) {
   clear_error();
   return parseval_launch(1, uh, prec, ndim, dims, rfft, nx, norm, out, as_stream(stream));
+}
+
+int ffhip_parseval_noalloc(int mode, const void* uh, int prec, int ndim, const int64_t* dims, int rfft, int64_t nx,
+                           double norm, double* out_dev, void* stream) {
+  clear_error();
+  FF_CHECK(mode == 0 || mode == 1, FFHIP_ERR_ARG, "mode must be 0 (parsevalsum2) or 1 (parsevalsum)");
+  FF_CHECK(uh && dims && out_dev, FFHIP_ERR_ARG, "null argument");
+  FF_CHECK(ndim >= 1 && ndim <= 4, FFHIP_ERR_ARG, "ndim must be 1..4");
+  FF_CHECK(valid_prec(prec), FFHIP_ERR_ARG, "bad precision %d", prec);
+  for (int i = 0; i < ndim; ++i) FF_CHECK(dims[i] >= 1, FFHIP_ERR_ARG, "dims[%d] = %lld", i, (long long)dims[i]);
+  const int64_t total = prod(dims, ndim);
+  if (rfft) FF_CHECK(dims[0] == nx / 2 + 1, FFHIP_ERR_ARG, "rfft layout needs dims[0] == nx/2+1");
+  int dev = 0;
+  FF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch[dev]) FF_TRY(red_scratch_setup(dev));  // ffhip_init not run here
+  double2* part = g_scratch[dev];
+  // grid-stride partial sums: one per workgroup, at most the kRedBlocks the scratch array holds
+  const int64_t want = (total + kRedThreads - 1) / kRedThreads;
+  const int blocks = (int)(want < kRedBlocks ? want : kRedBlocks);
+  FF_CHECK(blocks >= 1 && blocks <= kRedBlocks, FFHIP_ERR_ARG, "%d partial sums exceed the scratch array (%d)", blocks,
+           kRedBlocks);
+  hipStream_t st = as_stream(stream);
+  const int64_t nyq = rfft ? dims[0] - 1 : -1;
+#define LP(R, M)                                                                                                  \
+  hipLaunchKernelGGL((k_parseval<R, M>), dim3(blocks), dim3(kRedThreads), 0, st, (const cplx<R>*)uh, total, dims[0], \
+                     rfft, nyq, part)
+  if (prec == FFHIP_F32) {
+    if (mode == 0) LP(float, 0); else LP(float, 1);
+  } else {
+    if (mode == 0) LP(double, 0); else LP(double, 1);
+  }
+#undef LP
+  FF_LAUNCHED("k_parseval");
+  hipLaunchKernelGGL(k_dd_final_scaled, dim3(1), dim3(kRedThreads), 0, st, part, blocks, norm, out_dev);
+  FF_LAUNCHED("k_dd_final_scaled");
+  return FFHIP_OK;
 }
 
 int ffhip_etdrk4_create(ffhip_etdrk4** out, const double* Lre, const double* Lim, int ndim, const int64_t* dims,

@@ -48,5 +48,6 @@ __device__ __forceinline__ double2 block_dd_sum(double2 v) {
 int parseval_launch(int mode, const void* uh, int prec, int ndim, const int64_t* dims, int rfft, int64_t nx,
                     double norm, double* out, hipStream_t st);
 int red_buffer(double2** p);
+int red_scratch_setup(int device);  // device address of the allocation-free reduction's scratch array
 
 }  // namespace ffhip

@@ -9,6 +9,7 @@
 
 #include "common.h"
 #include "prof.h"
+#include "reduce.h"
 
 namespace ffhip {
 
@@ -144,7 +145,7 @@ int ffhip_init(int device) {
   FF_HIP(hipSetDevice(device));
   g_lds = 0;
   (void)max_lds_per_block();
-  return FFHIP_OK;
+  return red_scratch_setup(device);
 }
 
 int ffhip_malloc(void** ptr, size_t bytes) {

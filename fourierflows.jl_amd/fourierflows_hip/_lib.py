@@ -89,6 +89,7 @@ SIGNATURES = {
     "ffhip_rk4_update": [_vp, _vp, _vp, _vp, _vp, _vp, _dbl, _vp],
     "ffhip_parsevalsum2": [_vp, _int, _int, _i64p, _int, _i64, _dbl, _dp, _vp],
     "ffhip_parsevalsum": [_vp, _int, _int, _i64p, _int, _i64, _dbl, _dp, _vp],
+    "ffhip_parseval_noalloc": [_int, _vp, _int, _int, _i64p, _int, _i64, _dbl, _vp, _vp],
     "ffhip_op_create": [C.POINTER(_vp), _int, _int, _int, _i64p, _dp, _dbl, _dp, _int],
     "ffhip_op_destroy": [_vp],
     "ffhip_op_calcN": [_vp, _vp, _vp, _dbl, _vp],

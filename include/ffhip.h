@@ -188,6 +188,19 @@ int ffhip_parsevalsum2(const void* uh, int prec, int ndim, const int64_t* dims, 
                        int64_t nx, double norm, double* out, void* stream);
 int ffhip_parsevalsum(const void* uh, int prec, int ndim, const int64_t* dims, int rfft,
                       int64_t nx, double norm, double* out, void* stream);
+/* Allocation-free variant of the two sums above.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * mode 0 = parsevalsum2, 1 = parsevalsum, same arguments and arithmetic; the
+ * result (the same double) is written to device memory out_dev on `stream`
+ * with no host synchronisation, so no non-finite check is made -- the caller
+ * tests the value.  Partial sums use a fixed 16 KiB scratch array per GPU
+ * (1024 double-double partials; inputs of any size fit, the grid-stride pass
+ * never makes more partials); the device address is looked up by ffhip_init,
+ * or by the first call on a device ffhip_init did not set up.  Calls that
+ * overlap on one GPU race on that array. */
+int ffhip_parseval_noalloc(int mode, const void* uh, int prec, int ndim, const int64_t* dims,
+                           int rfft, int64_t nx, double norm, double* out_dev, void* stream);
 
 /* ---------------------------------------------------------------- built-in operators
  * The reference's calcN! is arbitrary user Julia.  A C ABI cannot compile

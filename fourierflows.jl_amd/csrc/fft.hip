@@ -418,6 +418,7 @@ static int run_rows_c2c(const cplx<R>* in, cplx<R>* out, int64_t nrows, const Ax
 template <typename R>
 static int run_cols_c2c(const cplx<R>* in, cplx<R>* out, int64_t ncols, int64_t stride, int64_t nslabs,
                         int64_t slab_dist, const Axis& ax, int dir, R scale, ffhip_plan* p, hipStream_t st) {
+  if (ax.fast) return fast_cols_c2c(p->prec, in, out, ncols, stride, nslabs, slab_dist, ax, dir, (double)scale, st);
   const int n = (int)ax.n;
   if (ax.lds) {
     const int C = (int)(ax.nb < ncols ? ax.nb : ncols);
@@ -442,6 +443,7 @@ static int run_cols_c2c(const cplx<R>* in, cplx<R>* out, int64_t ncols, int64_t 
 template <typename R>
 static int run_rows_r2c(const R* in, cplx<R>* out, int64_t nrows, ffhip_plan* p, hipStream_t st) {
   const Axis& ax = p->ax[0];
+  if (ax.fast) return fast_rows_r2c(p->prec, in, out, nrows, ax, p->rtw, st);
   const int m = (int)ax.n;
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(ax.tw);
   const cplx<R>* rtw = reinterpret_cast<const cplx<R>*>(p->rtw);
@@ -468,6 +470,7 @@ static int run_rows_r2c(const R* in, cplx<R>* out, int64_t nrows, ffhip_plan* p,
 template <typename R>
 static int run_rows_c2r(const cplx<R>* in, R* out, int64_t nrows, R scale, ffhip_plan* p, hipStream_t st) {
   const Axis& ax = p->ax[0];
+  if (ax.fast) return fast_rows_c2r(p->prec, in, out, nrows, ax, p->rtw, (double)scale, st);
   const int m = (int)ax.n;
   const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(ax.tw);
   const cplx<R>* rtw = reinterpret_cast<const cplx<R>*>(p->rtw);
@@ -665,8 +668,12 @@ static int create_plan(ffhip_plan** out, int kind, int ndim, const int64_t* n, i
   int rc = FFHIP_OK;
   const int64_t xlen = kind == 0 ? n[0] / 2 : n[0];
   rc = setup_axis(p->ax[0], xlen, prec, rows, force_global);
+  p->ax[0].fast = !force_global && kind == 0 && fast_rows_ok(xlen);
   if (rc == FFHIP_OK && kind == 0) rc = upload_roots(n[0], n[0] / 2 + 1, prec, &p->rtw);
-  for (int d = 1; d < ndim && rc == FFHIP_OK; ++d) rc = setup_axis(p->ax[d], n[d], prec, nx0, force_global);
+  for (int d = 1; d < ndim && rc == FFHIP_OK; ++d) {
+    rc = setup_axis(p->ax[d], n[d], prec, nx0, force_global);
+    p->ax[d].fast = !force_global && fast_cols_ok(n[d]);
+  }
   const int64_t spec = nx0 * (ndim > 1 ? n[1] : 1) * (ndim > 2 ? n[2] : 1) * batch;
   const size_t sbytes = (size_t)spec * cplx_size(prec);
   if (rc == FFHIP_OK && ndim > 1) rc = dev_alloc(&p->ws, sbytes);

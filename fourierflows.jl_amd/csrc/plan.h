@@ -16,11 +16,22 @@ struct Axis {
   void* tw = nullptr;
   bool lds = false;  // whole transform in one LDS tile
   int nb = 1;        // transforms (rows) or columns per block on the LDS path
+  bool fast = false; // power-of-two fast kernels (fft_fast.hip) run this axis
 };
 
 void unit_root(int64_t k, int64_t n, double* c, double* s);
 std::vector<int> factor_radices(int64_t n);
 int upload_roots(int64_t n, int64_t count, int prec, void** dptr);
+
+// fast kernels of power-of-two axes (fft_fast.hip); FFHIP_FFT_FAST=0 turns them off
+bool fast_rows_ok(int64_t m);  // r2c/c2r rows of half length m
+bool fast_cols_ok(int64_t n);  // c2c along a strided axis of length n
+int fast_rows_r2c(int prec, const void* in, void* out, int64_t nrows, const Axis& ax, const void* rtw,
+                  hipStream_t st);
+int fast_rows_c2r(int prec, const void* in, void* out, int64_t nrows, const Axis& ax, const void* rtw, double scale,
+                  hipStream_t st);
+int fast_cols_c2c(int prec, const void* in, void* out, int64_t ncols, int64_t stride, int64_t nslabs,
+                  int64_t slab_dist, const Axis& ax, int dir, double scale, hipStream_t st);
 
 }  // namespace ffhip
 

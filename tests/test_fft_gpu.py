@@ -110,8 +110,11 @@ def test_rfft_known_answers_3d_mixed_radix(ff):
     assert ou.isapprox(_host(back), f2, rtol=RTOL_FFT)
 
 
+# (ny, nx) / (nz, ny, nx): power-of-two rows with nx in 512..2048 and columns
+# of 64..2048 run the fast plan kernels, the rest the generic ones
 SHAPES = [(8,), (6,), (34,), (2048,), (6, 8), (64, 64), (30, 32), (34, 34), (18, 22),
-          (256, 128), (2048, 2048), (4096, 8), (16, 30, 32), (8, 10, 6), (64, 64, 64), (8, 8, 256)]
+          (256, 128), (2048, 2048), (4096, 8), (16, 30, 32), (8, 10, 6), (64, 64, 64), (8, 8, 256),
+          (512, 512), (1024, 1024), (128, 1024), (2048, 64), (32, 64, 512), (6, 512)]
 
 
 @pytest.mark.parametrize("prec", [64, 32])
@@ -191,6 +194,33 @@ def test_global_memory_path(ff, shape, monkeypatch):
     zo = _dev(np.zeros(shape, complex))
     cplan.forward(zo, _dev(z))
     assert ou.rel_l2(_host(zo), np.fft.fftn(z)) < 1e-13
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_fast_kernels_match_generic(ff, prec, monkeypatch):
+    """The power-of-two kernels and the generic Stockham kernels give the same
+    transforms (to roundoff) on a batched 2D plan and a 3D plan."""
+    import torch
+
+    T = np.float64 if prec == 64 else np.float32
+    tol = 1e-13 if prec == 64 else 2e-6
+    rng = np.random.default_rng(21)
+    for ns, batch in (((1024, 256), 3), ((512, 128, 64), 1)):
+        shape = (batch,) + tuple(reversed(ns))
+        f = _dev(rng.standard_normal(shape).astype(T))
+        outs = []
+        for fast in ("1", "0"):
+            monkeypatch.setenv("FFHIP_FFT_FAST", fast)
+            plan = ff.Plan(ns, T, "rfft", batch=batch)
+            monkeypatch.delenv("FFHIP_FFT_FAST")
+            fh = torch.zeros(shape[:-1] + (ns[0] // 2 + 1,), dtype=torch.complex128 if prec == 64 else
+                             torch.complex64, device="cuda")
+            plan.forward(fh, f)
+            back = torch.zeros_like(f)
+            plan.inverse(back, fh)
+            outs.append((_host(fh), _host(back)))
+        assert ou.rel_l2(outs[0][0], outs[1][0]) < tol
+        assert ou.rel_l2(outs[0][1], outs[1][1]) < tol
 
 
 def test_batched_plan(ff):

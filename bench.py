@@ -62,6 +62,47 @@ def time_steps(ff, torch, prob, steps, warmup):
     return e0.elapsed_time(e1) / 1000.0
 
 
+GOLDEN = os.path.join(ROOT, "tests", "golden", "vort2048_etdrk4_2000.npz")
+
+
+def accuracy_vs_oracle(ff, torch):
+    """The workload's full 2000-step run in FP32 and in FP64 against the FP64
+    CPU oracle's fixture (written by tests/golden/make_golden.py): the largest
+    relative error of energy and enstrophy over the steps saved every 200, the
+    relative L2 error of the |kx|, |l| <= 32 block of the spectral vorticity
+    after 1000 and 2000 steps, and the FP32 field against the FP64 field."""
+    import numpy as np
+
+    if not os.path.exists(GOLDEN):
+        return None
+    gd = np.load(GOLDEN)
+    K = int(gd["K"])
+    steps = [int(x) for x in gd["steps"]]
+    out, finals = {"steps": steps[-1], "reference": "FP64 CPU oracle (numpy/pocketfft)"}, {}
+    for T in (32, 64):
+        p = make_problem(ff, T)
+        z0 = ff.vorticity_ic(p.grid, seed=int(gd["seed"]))
+        p.sol.copy_(z0)
+        e_err = z_err = 0.0
+        blocks = {}
+        for i, k in enumerate(steps):
+            if k > 0:
+                ff.stepforward(p, k - steps[i - 1])
+            e, z = p.eqn.calcN.energy_enstrophy(p.sol)
+            e_err = max(e_err, abs(e - gd["energy"][i]) / gd["energy"][i])
+            z_err = max(z_err, abs(z - gd["enstrophy"][i]) / gd["enstrophy"][i])
+            if f"zh_low_{k}" in gd.files:
+                zh = p.sol.to(torch.complex128).cpu().numpy()
+                blk = np.concatenate([zh[: K + 1, : K + 1], zh[-K:, : K + 1]], axis=0)
+                ref = gd[f"zh_low_{k}"]
+                blocks[str(k)] = float(np.linalg.norm(blk - ref) / np.linalg.norm(ref))
+        finals[T] = p.sol.to(torch.complex128)
+        out[f"fp{T}"] = {"energy": e_err, "enstrophy": z_err, "low_mode_block": blocks}
+    out["fp32_field_vs_fp64_field"] = float(torch.linalg.vector_norm(finals[32] - finals[64]) /
+                                            torch.linalg.vector_norm(finals[64]))
+    return out
+
+
 def kernel_profile(ff, prob, steps):
     """Per-kernel averages over `steps` event-bracketed steps."""
     from fourierflows_hip import _lib
@@ -132,7 +173,11 @@ def ensemble_timing(ff, torch, nsteps=500, dt=0.002):
             "steps": nsteps, "seconds": round(t, 4), "ensemble_steps_per_s": round(nsteps / t, 2),
             "realization_steps_per_s": round(len(members) * nsteps / t, 1),
             "grid_points_per_step": int(sum(n * n for (n, _, _, _) in members)),
-            "variant": {"threads": threads, "points_rule": prule, "tuned_ms_per_step": round(tuned_ms, 4)}}
+            "variant": {"threads": threads,
+                        "points_per_thread": {f"{64 << c}": {"cols": 16 if (prule >> c) & 1 else 8,
+                                                             "rows": 16 if (prule >> (8 + c)) & 1 else 8}
+                                              for c in range(5) if (64 << c) in (128, 256, 512)},
+                        "tuned_ms_per_step": round(tuned_ms, 4)}}
 
 
 def tracer_timing(ff, torch, n=256, nsteps=300, dt=0.0025, kappa=1e-3):
@@ -164,6 +209,23 @@ def tracer_timing(ff, torch, n=256, nsteps=300, dt=0.0025, kappa=1e-3):
             "steps_per_s": round(nsteps / tf, 2), "forward_s": round(tf, 4), "gradient_s": round(tg, 4),
             "gradient_over_forward": round(tg / tf, 3),
             "note": "dt=0.01 is outside RK4's stability region for this flow at 256^3 (growth 2.4 per step)"}
+
+
+def vort4096_timing(ff, torch, steps=50):
+    """The same hyperviscous vorticity problem at 4096² in FP32 (the size the
+    reference's docs time: docs/src/assets/CPUversusGPU.jl:11, 23.724 s per 100
+    GPU steps): fused steps/s over `steps` steps."""
+    import numpy as np
+
+    n = 4096
+    p = ff.TwoDVorticityProblem(nx=n, nu_n=nu8_for(n, DT), n_hyper=4, mu=MU, dt=DT, T=np.float32)
+    if not p.eqn.calcN.fused:
+        return None
+    p.sol.copy_(ff.vorticity_ic(p.grid, seed=1))
+    t = time_steps(ff, torch, p, steps, 5)
+    return {"workload": f"2D vorticity {n}^2 FP32, nu8|k|^8 + mu=0.01, dt=1e-3", "steps": steps,
+            "steps_per_s": round(steps / t, 2), "ms_per_step": round(1e3 * t / steps, 4),
+            "reference_published_gpu_steps_per_s": round(100 / 23.724, 3)}
 
 
 def fft_plan_timing(ff, torch, n=N, reps=50):
@@ -328,19 +390,8 @@ def main():
     print(f"# FP64: {sps64:.1f} steps/s", file=sys.stderr, flush=True)
     prof64 = kernel_profile(ff, p64, a.profile_steps)
 
-    # accuracy of the FP32 run against the FP64 run from the same start, same step count
-    p32b = make_problem(ff, 32)
-    p32b.sol.copy_(zh64.to(torch.complex64))
-    p64b = make_problem(ff, 64)
-    p64b.sol.copy_(zh64)
-    nacc = min(a.steps, 500)
-    ff.stepforward(p32b, nacc)
-    ff.stepforward(p64b, nacc)
-    torch.cuda.synchronize()
-    x32 = p32b.sol.to(torch.complex128)
-    err = float(torch.linalg.vector_norm(x32 - p64b.sol) / torch.linalg.vector_norm(p64b.sol))
-    e32, z32 = p32b.eqn.calcN.energy_enstrophy(p32b.sol)
-    e64, z64 = p64b.eqn.calcN.energy_enstrophy(p64b.sol)
+    accuracy = accuracy_vs_oracle(ff, torch)
+    print(f"# accuracy: {accuracy}", file=sys.stderr, flush=True)
 
     # dominant kernel of the FP32 step: the one with the largest total time
     dom_name, dom = max(prof32.items(), key=lambda kv: kv[1]["total_ms"])
@@ -362,6 +413,8 @@ def main():
     if not a.no_ensemble:
         ens = ensemble_timing(ff, torch)
         print(f"# ensemble: {ens}", file=sys.stderr, flush=True)
+    big = vort4096_timing(ff, torch)
+    print(f"# 4096^2: {big}", file=sys.stderr, flush=True)
     fft = None
     if not a.no_fft:
         fft = fft_plan_timing(ff, torch)
@@ -385,12 +438,12 @@ def main():
         "config": {"workload": "2D vorticity 2048^2 ETDRK4, nu8|k|^8 + mu=0.01, dt=1e-3, 2/3 dealiasing",
                    "nx": N, "ny": N, "stepper": "ETDRK4", "dt": DT, "parallelism": "single GPU"},
         "fp64_steps_per_s": round(sps64, 2), "fp32_over_fp64": round(sps32 / sps64, 3),
-        "fp32_rel_l2_vs_fp64": {"steps": nacc, "field": err, "energy": abs(e32 - e64) / abs(e64),
-                                "enstrophy": abs(z32 - z64) / abs(z64)},
+        "accuracy_vs_fp64_oracle": accuracy,
         "adjoint": adj,
         "ensemble": ens,
         "tracer": trc,
         "fft_plan": fft,
+        "vorticity_4096": big,
         "step_kept_band_GB": round(step_bytes / 1e9, 4),
         "step_kept_band_GBps": round(step_bytes / (t32 / a.steps) / 1e9, 1),
         "step_algorithmic_GB": round(70 * (N // 2 + 1) * N * 8 / 1e9, 4),

@@ -72,9 +72,14 @@ constexpr int vpts_row(int n) { return n >= 256 ? 16 : 8; }
 // registers (measured on MI355X: 512^2 forward 46.0 -> 37.4 ms per 500
 // steps).  Where 8 adds a pass (1024 and 2048: 4 vs 3) it stays at 16
 // (2048^2 measured 34.5 vs 27.5 us per stage).
+#ifndef FF_COL_P8_ALWAYS
+#define FF_COL_P8_ALWAYS 0  // 1: FP32 columns take 8 points per thread at every size (measurement switch)
+#endif
 template <typename R, int N>
 constexpr int vcol_pts() {
-  return sizeof(R) == 4 && N >= 256 && TeamSched<N, 8>::NPASS == TeamSched<N, 16>::NPASS ? 8 : vpts_col(N);
+  return sizeof(R) == 4 && N >= 256 && (FF_COL_P8_ALWAYS || TeamSched<N, 8>::NPASS == TeamSched<N, 16>::NPASS)
+             ? 8
+             : vpts_col(N);
 }
 // The same rule for the single-team row passes, forward and adjoint (512^2
 // forward, measured: 37.3 -> 33.0 ms per 500 steps).

@@ -298,6 +298,9 @@ static int dispatch_col(const VortGeom& g, const VortColArgs& a, hipStream_t st)
     case 2048:
       if (col3_on(sizeof(R) == 4)) return launch_col3<R, 2048, STAGE>(g, a, st);
       return launch_col<R, 2048, STAGE>(g, a, st);
+    case 4096:  // FP32 only: an FP64 4096-point column team needs more LDS than a workgroup has
+      if constexpr (sizeof(R) == 4) return launch_col<R, 4096, STAGE>(g, a, st);
+      return fail(FFHIP_ERR_UNSUPPORTED, "fused column kernel: FP64 ny = %d", g.ny);
     default: return fail(FFHIP_ERR_UNSUPPORTED, "fused column kernel: ny = %d", g.ny);
   }
 }
@@ -315,6 +318,9 @@ static int dispatch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st)
     case 2048:
       if (row3_on(sizeof(R) == 4)) return launch_row3<R, 2048>(g, a, st);
       return launch_row<R, 2048>(g, a, st);
+    case 4096:  // FP32 only (two paired 4096-point rows per workgroup: 139 KB of LDS)
+      if constexpr (sizeof(R) == 4) return launch_row<R, 4096>(g, a, st);
+      return fail(FFHIP_ERR_UNSUPPORTED, "fused row kernel: FP64 nx = %d", g.nx);
     default: return fail(FFHIP_ERR_UNSUPPORTED, "fused row kernel: nx = %d", g.nx);
   }
 }
@@ -360,7 +366,8 @@ int fused_create(ffhip_op* op) {
   const char* dis = getenv("FFHIP_DISABLE_FUSED");
   if (dis && atoi(dis) != 0) return FFHIP_OK;
   const int64_t nx = op->n[0], ny = op->n[1];
-  if (!pow2(nx) || !pow2(ny) || nx < 64 || ny < 64 || nx > 2048 || ny > 2048) return FFHIP_OK;
+  const int64_t nmax = op->prec == FFHIP_F32 ? 4096 : 2048;  // FP32 kernels go to 4096 points
+  if (!pow2(nx) || !pow2(ny) || nx < 64 || ny < 64 || nx > nmax || ny > nmax) return FFHIP_OK;
   if (!(op->af > 0)) return FFHIP_OK;  // the fused kernels assume the Nyquist modes are aliased
   VortGeom g;
   g.nx = (int)nx;
@@ -493,15 +500,17 @@ static int vrun(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, hipSt
     return FFHIP_OK;
   };
   int64_t left = nsteps;
-  // Whole chunks of kVortChunk steps replay a captured graph (no launch gaps
-  // between the 80 kernels).  The first step runs eagerly before any capture
-  // (so every kernel's LDS attribute is set outside it), the last eagerly
-  // because it differs.  Profiled runs stay eager: they time each launch.
-  const bool graphs = !g_prof_on && nsteps >= kVortChunk + 2 && vort_graphs_on();
-  if (graphs) {
-    FF_TRY(step(st, false));
+  // Whole chunks of kVortChunk steps replay a captured graph.  The graph is
+  // captured at the first run of the operator with this stepper, whatever its
+  // length, right after one eager step (so every kernel's LDS attribute is
+  // set outside the capture); later runs find it ready.  The last step runs
+  // eagerly because it differs.  Profiled runs stay eager: they time each
+  // launch.
+  const bool graphs = !g_prof_on && vort_graphs_on();
+  if (graphs && left >= 1 && (!s->exec || s->graph_packed != ts->packed || s->graph_filter != (ts->filter != nullptr))) {
+    FF_TRY(step(st, left == 1));
     --left;
-    if (!s->exec || s->graph_packed != ts->packed || s->graph_filter != (ts->filter != nullptr)) {
+    {
       drop_graph(s);
       if (!s->cap) FF_HIP(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
       FF_HIP(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
@@ -522,9 +531,11 @@ static int vrun(ffhip_op* op, ffhip_etdrk4* ts, void* sol, int64_t nsteps, hipSt
       }
       s->graph_packed = ts->packed;
       s->graph_filter = ts->filter != nullptr;
+      FF_HIP(hipGraphUpload(s->exec, st));  // the first replay then carries no upload
     }
-    for (; left > kVortChunk; left -= kVortChunk) FF_HIP(hipGraphLaunch(s->exec, st));
   }
+  if (graphs)
+    for (; left > kVortChunk; left -= kVortChunk) FF_HIP(hipGraphLaunch(s->exec, st));
   for (int64_t k = 0; k < left; ++k) FF_TRY(step(st, k + 1 == left));
   const int64_t total = (int64_t)g.ny * g.nkr;
   hipLaunchKernelGGL((k_vexport<R>), dim3(blocks_for(total, 256, 4096)), dim3(256), 0, st,

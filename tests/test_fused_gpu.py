@@ -66,7 +66,7 @@ def test_fused_fp64_matches_oracle(ff, n, ny, Ly):
 
 @pytest.mark.parametrize("n,ny,Ly", [(64, 64, 2 * np.pi), (256, 128, 4 * np.pi), (128, 512, 2 * np.pi),
                                      (512, 512, 2 * np.pi), (1024, 1024, 2 * np.pi), (2048, 1024, np.pi),
-                                     (1024, 2048, 2 * np.pi)])
+                                     (1024, 2048, 2 * np.pi), (4096, 64, np.pi / 16), (64, 4096, 32 * np.pi)])
 def test_fused_fp32_matches_oracle(ff, n, ny, Ly):
     """The FP32 fused passes (paired transforms up to 1024 in the column pass,
     paired inverses in every row pass) against the FP64 oracle: relative L2
@@ -81,6 +81,25 @@ def test_fused_fp32_matches_oracle(ff, n, ny, Ly):
     prob.sol.copy_(_dev(zh0.astype(np.complex64)))
     ff.stepforward(prob, steps)
     err = ou.rel_l2(_host(prob.sol).astype(np.complex128), ref)
+    assert err < 1e-5, err
+
+
+def test_fused_fp32_4096_square(ff):
+    """4096² FP32 (fused up to 4096 points per axis in FP32; FP64 4096² runs
+    the plan-based path): a few steps against the FP64 plan-based run."""
+    import torch
+
+    n, dt = 4096, 1e-3
+    nu8 = oph.hyperviscosity_for(n, dt)
+    p32 = ff.TwoDVorticityProblem(nx=n, nu_n=nu8, n_hyper=4, mu=0.01, dt=dt, T=np.float32)
+    p64 = ff.TwoDVorticityProblem(nx=n, nu_n=nu8, n_hyper=4, mu=0.01, dt=dt, T=np.float64)
+    assert p32.eqn.calcN.fused and not p64.eqn.calcN.fused
+    z0 = ff.vorticity_ic(p64.grid, seed=3)
+    p64.sol.copy_(z0)
+    p32.sol.copy_(z0.to(torch.complex64))
+    ff.stepforward(p64, 4)
+    ff.stepforward(p32, 4)
+    err = float(torch.linalg.vector_norm(p32.sol.to(torch.complex128) - p64.sol) / torch.linalg.vector_norm(p64.sol))
     assert err < 1e-5, err
 
 

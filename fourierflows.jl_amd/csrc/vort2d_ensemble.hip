@@ -11,10 +11,17 @@
 // keeps the team barriers uniform.  Items are ordered largest grid first.
 //
 // Steps are captured once into a HIP graph (10 steps = 80 kernel nodes) and
-// replayed.  The tile/radix variant (BD in {128, 256}; P = 8 everywhere or
-// P = 16 from n = 256 up) is either the single-grid default or picked by
-// timing every valid variant on the ensemble itself.
+// replayed.  A variant fixes the workgroup size BD in {128, 256} and, per
+// pass (rows, columns), the grid size from which a transform takes 16 points
+// per thread instead of 8 (128, 256, 512 or never) -- so each member size
+// gets its radix schedule (8^a 2^b ... or 16^a 8^b ...) and tile shape (BD P/n
+// rows or columns per workgroup).  The threshold is a template parameter: a
+// kernel holds only the bodies it runs, so small members keep the occupancy
+// of their 8-point bodies.  The default is the single-grid rule (16 points
+// from n = 256 up); the autotuner times all 32 variants on the ensemble
+// itself and keeps the fastest.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -33,12 +40,21 @@ struct EnsMember {  // device-resident descriptor of one member
 
 constexpr int kEnsChunk = 10;  // steps per captured graph
 
-constexpr int ens_pts(int n, int prule) { return (prule == 1 && n >= 256) ? 16 : 8; }
-constexpr bool ens_ok(int n, int prule, int bd) { return n / ens_pts(n, prule) <= bd; }
+// points per thread of size n when sizes from t up take 16
+constexpr int ens_pts(int n, int t) { return n >= t ? 16 : 8; }
+constexpr bool ens_ok(int n, int t, int bd) { return n / ens_pts(n, t) <= bd; }
+constexpr int kEnsThresholds[4] = {128, 256, 512, 2048};  // 2048: 8 points at every ensemble size
+constexpr int kEnsDefaultThr = 256;                       // the single-grid rule
+// per-size mask of a threshold (bit s: size 64 << s takes 16 points)
+constexpr int ens_mask(int t) {
+  int m = 0;
+  for (int s = 0; s < 5; ++s)
+    if ((64 << s) >= t) m |= 1 << s;
+  return m;
+}
 
-template <typename R, int BD, int PRULE, int N, int STAGE>
+template <typename R, int BD, int P, int N, int STAGE>
 __device__ __forceinline__ void ens_col_n(const VortGeom& g, const VortColArgs& a, int first, unsigned char* sm) {
-  constexpr int P = ens_pts(N, PRULE);
   constexpr int T = N / P;
   if constexpr (T <= BD) {
     constexpr int CPB = BD / T;
@@ -52,9 +68,8 @@ __device__ __forceinline__ void ens_col_n(const VortGeom& g, const VortColArgs& 
   }
 }
 
-template <typename R, int BD, int PRULE, int N>
+template <typename R, int BD, int P, int N>
 __device__ __forceinline__ void ens_row_n(const VortGeom& g, const VortRowArgs& a, int first, unsigned char* sm) {
-  constexpr int P = ens_pts(N, PRULE);
   constexpr int T = N / P;
   if constexpr (T <= BD) {
     constexpr int RPB = BD / T;
@@ -67,33 +82,35 @@ __device__ __forceinline__ void ens_row_n(const VortGeom& g, const VortRowArgs& 
   }
 }
 
-template <typename R, int BD, int PRULE, int STAGE>
+// The member's size picks the body (uniform over the workgroup: every item
+// is one member); TH (threshold) fixes each size's points per thread.
+template <typename R, int BD, int TH, int STAGE>
 __global__ void __launch_bounds__(BD) k_ens_col(const EnsMember* __restrict__ mem, const int2* __restrict__ items) {
   extern __shared__ unsigned char vsm[];
   const int2 it = items[blockIdx.x];
   const VortGeom g = mem[it.x].g;
   const VortColArgs a = mem[it.x].ca;
   switch (g.ny) {
-    case 64: ens_col_n<R, BD, PRULE, 64, STAGE>(g, a, it.y, vsm); break;
-    case 128: ens_col_n<R, BD, PRULE, 128, STAGE>(g, a, it.y, vsm); break;
-    case 256: ens_col_n<R, BD, PRULE, 256, STAGE>(g, a, it.y, vsm); break;
-    case 512: ens_col_n<R, BD, PRULE, 512, STAGE>(g, a, it.y, vsm); break;
-    default: ens_col_n<R, BD, PRULE, 1024, STAGE>(g, a, it.y, vsm); break;
+    case 64: ens_col_n<R, BD, ens_pts(64, TH), 64, STAGE>(g, a, it.y, vsm); break;
+    case 128: ens_col_n<R, BD, ens_pts(128, TH), 128, STAGE>(g, a, it.y, vsm); break;
+    case 256: ens_col_n<R, BD, ens_pts(256, TH), 256, STAGE>(g, a, it.y, vsm); break;
+    case 512: ens_col_n<R, BD, ens_pts(512, TH), 512, STAGE>(g, a, it.y, vsm); break;
+    default: ens_col_n<R, BD, ens_pts(1024, TH), 1024, STAGE>(g, a, it.y, vsm); break;
   }
 }
 
-template <typename R, int BD, int PRULE>
+template <typename R, int BD, int TH>
 __global__ void __launch_bounds__(BD) k_ens_row(const EnsMember* __restrict__ mem, const int2* __restrict__ items) {
   extern __shared__ unsigned char vsm[];
   const int2 it = items[blockIdx.x];
   const VortGeom g = mem[it.x].g;
   const VortRowArgs a = mem[it.x].ra;
   switch (g.nx) {
-    case 64: ens_row_n<R, BD, PRULE, 64>(g, a, it.y, vsm); break;
-    case 128: ens_row_n<R, BD, PRULE, 128>(g, a, it.y, vsm); break;
-    case 256: ens_row_n<R, BD, PRULE, 256>(g, a, it.y, vsm); break;
-    case 512: ens_row_n<R, BD, PRULE, 512>(g, a, it.y, vsm); break;
-    default: ens_row_n<R, BD, PRULE, 1024>(g, a, it.y, vsm); break;
+    case 64: ens_row_n<R, BD, ens_pts(64, TH), 64>(g, a, it.y, vsm); break;
+    case 128: ens_row_n<R, BD, ens_pts(128, TH), 128>(g, a, it.y, vsm); break;
+    case 256: ens_row_n<R, BD, ens_pts(256, TH), 256>(g, a, it.y, vsm); break;
+    case 512: ens_row_n<R, BD, ens_pts(512, TH), 512>(g, a, it.y, vsm); break;
+    default: ens_row_n<R, BD, ens_pts(1024, TH), 1024>(g, a, it.y, vsm); break;
   }
 }
 
@@ -102,7 +119,8 @@ __global__ void __launch_bounds__(BD) k_ens_row(const EnsMember* __restrict__ me
 using namespace ffhip;
 
 struct EnsVariant {
-  int bd = 128, prule = 1;
+  int bd = 128;
+  int tcol = kEnsDefaultThr, trow = kEnsDefaultThr;  // sizes from these up take 16 points per thread
 };
 
 struct ffhip_ensemble {
@@ -133,7 +151,7 @@ static bool pow2i(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 static bool variant_ok(const ffhip_ensemble* e, EnsVariant v) {
   for (const auto& g : e->geo)
-    if (!ens_ok(g.nx, v.prule, v.bd) || !ens_ok(g.ny, v.prule, v.bd)) return false;
+    if (!ens_ok(g.nx, v.trow, v.bd) || !ens_ok(g.ny, v.tcol, v.bd)) return false;
   return true;
 }
 
@@ -149,10 +167,10 @@ static int build_items(ffhip_ensemble* e, EnsVariant v) {
   const size_t cs = cplx_size(e->prec);
   for (int m : order) {
     const VortGeom& g = e->geo[m];
-    const int Py = ens_pts(g.ny, v.prule), Px = ens_pts(g.nx, v.prule);
+    const int Py = ens_pts(g.ny, v.tcol), Px = ens_pts(g.nx, v.trow);
     const int cpb = v.bd / (g.ny / Py), rpb = v.bd / (g.nx / Px);
-    FF_CHECK(cpb >= 1 && rpb >= 1, FFHIP_ERR_UNSUPPORTED, "variant %d/%d cannot hold a %d x %d member", v.bd,
-             v.prule, g.nx, g.ny);
+    FF_CHECK(cpb >= 1 && rpb >= 1, FFHIP_ERR_UNSUPPORTED, "variant %d/%d/%d cannot hold a %d x %d member", v.bd,
+             v.tcol, v.trow, g.nx, g.ny);
     for (int c = 0; c < g.TX * kVCB; c += cpb) cols.push_back(make_int2(m, c));
     for (int y = 0; y < g.ny; y += rpb) rows.push_back(make_int2(m, y));
     lc = std::max(lc, ((size_t)cpb * (size_t)team_slice(g.ny + g.ny / 16, (int)cs, cpb) + tw_slots_rt(g.ny, Py)) * cs);
@@ -190,14 +208,14 @@ static int set_lds(K kern, size_t bytes) {
 
 // which: 0..4 column stage, 5 row pass.  attr_only: set the kernel's LDS
 // limit without launching (done outside any capture).
-template <typename R, int BD, int PRULE>
+template <typename R, int BD, int TC, int TR>
 static int ens_pass(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only) {
   const EnsMember* mem = reinterpret_cast<const EnsMember*>(e->dmem);
   const int2* items = reinterpret_cast<const int2*>(e->items);
   static const char* names[6] = {"ens_col_stage0", "ens_col_stage1", "ens_col_stage2",
                                  "ens_col_stage3", "ens_col_stage4", "ens_row"};
   if (which == 5) {
-    auto k = k_ens_row<R, BD, PRULE>;
+    auto k = k_ens_row<R, BD, TR>;
     if (attr_only) return set_lds(k, e->lds_row);
     prof_launch(names[5], e->bytes[5], k, dim3(e->nrow), dim3(BD), e->lds_row, st, mem, items + e->ncol);
     FF_LAUNCHED("k_ens_row");
@@ -205,11 +223,11 @@ static int ens_pass(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only
   }
   void (*k)(const EnsMember*, const int2*) = nullptr;
   switch (which) {
-    case 0: k = k_ens_col<R, BD, PRULE, 0>; break;
-    case 1: k = k_ens_col<R, BD, PRULE, 1>; break;
-    case 2: k = k_ens_col<R, BD, PRULE, 2>; break;
-    case 3: k = k_ens_col<R, BD, PRULE, 3>; break;
-    default: k = k_ens_col<R, BD, PRULE, 4>; break;
+    case 0: k = k_ens_col<R, BD, TC, 0>; break;
+    case 1: k = k_ens_col<R, BD, TC, 1>; break;
+    case 2: k = k_ens_col<R, BD, TC, 2>; break;
+    case 3: k = k_ens_col<R, BD, TC, 3>; break;
+    default: k = k_ens_col<R, BD, TC, 4>; break;
   }
   if (attr_only) return set_lds(k, e->lds_col);
   prof_launch(names[which], e->bytes[which], k, dim3(e->ncol), dim3(BD), e->lds_col, st, mem, items);
@@ -219,10 +237,24 @@ static int ens_pass(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only
 
 template <typename R>
 static int ens_pass_v(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only) {
-  const EnsVariant v = e->var;
-  if (v.bd == 128)
-    return v.prule ? ens_pass<R, 128, 1>(e, which, st, attr_only) : ens_pass<R, 128, 0>(e, which, st, attr_only);
-  return v.prule ? ens_pass<R, 256, 1>(e, which, st, attr_only) : ens_pass<R, 256, 0>(e, which, st, attr_only);
+  // column passes use tcol, the row pass trow: instantiate each pass with its own threshold only
+  const EnsVariant& v = e->var;
+  const int t = which == 5 ? v.trow : v.tcol;
+#define FF_ENS_T(BD, T)                                                                                      \
+  (which == 5 ? ens_pass<R, BD, 2048, T>(e, which, st, attr_only) : ens_pass<R, BD, T, 2048>(e, which, st, attr_only))
+#define FF_ENS_BD(BD)                                                           \
+  switch (t) {                                                                  \
+    case 128: return FF_ENS_T(BD, 128);                                         \
+    case 256: return FF_ENS_T(BD, 256);                                         \
+    case 512: return FF_ENS_T(BD, 512);                                         \
+    default: return FF_ENS_T(BD, 2048);                                         \
+  }
+  if (v.bd == 128) {
+    FF_ENS_BD(128)
+  }
+  FF_ENS_BD(256)
+#undef FF_ENS_BD
+#undef FF_ENS_T
 }
 
 static int ens_launch(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only = false) {
@@ -268,6 +300,7 @@ static int capture_graph(ffhip_ensemble* e) {
     return fail(FFHIP_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ee));
   }
   e->graph_gen = e->items_gen;
+  FF_HIP(hipGraphUpload(e->exec, e->cap));  // the first replay then carries no upload
   return FFHIP_OK;
 }
 
@@ -277,50 +310,63 @@ static int set_variant_attrs(ffhip_ensemble* e) {
   return FFHIP_OK;
 }
 
-// time every valid variant on the ensemble's own buffers (their contents are
-// overwritten by the import that follows) and keep the fastest
-static int autotune(ffhip_ensemble* e, hipStream_t st) {
-  const EnsVariant cands[4] = {{128, 1}, {128, 0}, {256, 1}, {256, 0}};
-  hipEvent_t a, b;
-  FF_HIP(hipEventCreate(&a));
-  FF_HIP(hipEventCreate(&b));
+// ms per step of variant v on the ensemble's own buffers (their contents are
+// overwritten by the import that follows), or a negative value if v is invalid
+static int time_variant(ffhip_ensemble* e, EnsVariant v, hipStream_t st, hipEvent_t a, hipEvent_t b, double* ms) {
+  *ms = -1;
+  if (!variant_ok(e, v)) return FFHIP_OK;
+  FF_HIP(hipStreamSynchronize(st));  // the item table is replaced below
+  FF_TRY(build_items(e, v));
+  FF_TRY(set_variant_attrs(e));
+  for (int k = 0; k < 2; ++k) FF_TRY(ens_step(e, st));
+  FF_HIP(hipEventRecord(a, st));
+  for (int k = 0; k < 5; ++k) FF_TRY(ens_step(e, st));
+  FF_HIP(hipEventRecord(b, st));
+  FF_HIP(hipEventSynchronize(b));
+  float t = 0;
+  FF_HIP(hipEventElapsedTime(&t, a, b));
+  *ms = t / 5.0;
+  return FFHIP_OK;
+}
+
+// Every (BD, column threshold, row threshold) that fits the members; the
+// fastest is kept.
+static int autotune_search(ffhip_ensemble* e, hipStream_t st, hipEvent_t a, hipEvent_t b) {
   double best = 1e300;
   EnsVariant bestv = e->var;
-  int rc = FFHIP_OK;
-  for (const EnsVariant& v : cands) {
-    if (!variant_ok(e, v)) continue;
-    if (hipStreamSynchronize(st) != hipSuccess) {  // the item table is replaced below
-      rc = fail(FFHIP_ERR_HIP, "autotune: stream synchronisation failed");
-      break;
-    }
-    rc = build_items(e, v);
-    if (rc != FFHIP_OK) break;
-    rc = set_variant_attrs(e);
-    for (int k = 0; k < 2 && rc == FFHIP_OK; ++k) rc = ens_step(e, st);
-    if (rc != FFHIP_OK) break;
-    (void)hipEventRecord(a, st);
-    for (int k = 0; k < 5 && rc == FFHIP_OK; ++k) rc = ens_step(e, st);
-    (void)hipEventRecord(b, st);
-    if (rc != FFHIP_OK) break;
-    if (hipEventSynchronize(b) != hipSuccess) {
-      rc = fail(FFHIP_ERR_HIP, "autotune timing failed");
-      break;
-    }
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, a, b);
-    if (ms / 5 < best) {
-      best = ms / 5;
-      bestv = v;
-    }
-  }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  FF_TRY(rc);
+  for (int bd : {128, 256})
+    for (int tc : kEnsThresholds)
+      for (int tr : kEnsThresholds) {
+        EnsVariant v;
+        v.bd = bd;
+        v.tcol = tc;
+        v.trow = tr;
+        double ms = 0;
+        FF_TRY(time_variant(e, v, st, a, b, &ms));
+        if (ms >= 0 && ms < best) {
+          best = ms;
+          bestv = v;
+        }
+      }
+  FF_CHECK(best < 1e300, FFHIP_ERR_UNSUPPORTED, "autotune: no variant fits the ensemble");
   FF_HIP(hipStreamSynchronize(st));
   FF_TRY(build_items(e, bestv));
   e->tuned_ms = best;
   e->tuned = 1;
   return FFHIP_OK;
+}
+
+static int autotune(ffhip_ensemble* e, hipStream_t st) {
+  hipEvent_t a, b;
+  FF_HIP(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail(FFHIP_ERR_HIP, "autotune: hipEventCreate failed");
+  }
+  const int rc = autotune_search(e, st, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
 }
 
 static void ens_free(ffhip_ensemble* e) {
@@ -480,10 +526,17 @@ int ffhip_ensemble_create(ffhip_ensemble** out, int prec, int count, const int64
   }
   if (rc == FFHIP_OK) {
     EnsVariant v;  // the single-grid kernels' radix schedule
-    v.bd = 128;
-    v.prule = 1;
     if (!variant_ok(e, v)) v.bd = 256;
-    rc = build_items(e, v);
+    // FFHIP_ENS_VARIANT="bd,tcol,trow" (thresholds 128, 256, 512 or 2048) fixes the untuned variant
+    if (const char* ev = getenv("FFHIP_ENS_VARIANT")) {
+      EnsVariant f;
+      auto thr_ok = [](int t) { return t == 128 || t == 256 || t == 512 || t == 2048; };
+      if (sscanf(ev, "%d,%d,%d", &f.bd, &f.tcol, &f.trow) != 3 || (f.bd != 128 && f.bd != 256) ||
+          !thr_ok(f.tcol) || !thr_ok(f.trow) || !variant_ok(e, f))
+        rc = fail(FFHIP_ERR_ARG, "FFHIP_ENS_VARIANT=%s is not a valid variant for this ensemble", ev);
+      v = f;
+    }
+    if (rc == FFHIP_OK) rc = build_items(e, v);
   }
   if (rc != FFHIP_OK) {
     ens_free(e);
@@ -527,7 +580,7 @@ int ffhip_ensemble_info(const ffhip_ensemble* e, int* threads, int* prule, doubl
   clear_error();
   FF_CHECK(e, FFHIP_ERR_ARG, "null ensemble");
   if (threads) *threads = e->var.bd;
-  if (prule) *prule = e->var.prule;
+  if (prule) *prule = ens_mask(e->var.tcol) | (ens_mask(e->var.trow) << 8);
   if (ms_per_step) *ms_per_step = e->tuned ? e->tuned_ms : 0.0;
   return FFHIP_OK;
 }

@@ -261,7 +261,10 @@ class VorticityEnsemble:
         self.step += int(nsteps)
 
     def info(self):
-        """(threads per workgroup, points-per-thread rule, tuned ms per step or 0)."""
+        """(threads per workgroup, points-per-thread masks, tuned ms per step or 0).
+
+        Mask bit s (grid size 64 << s) set: 16 points per thread in that
+        size's column passes, else 8; bit 8 + s the same for its row passes."""
         t, p, ms = C.c_int(), C.c_int(), C.c_double()
         _lib.call("ffhip_ensemble_info", self._h, C.byref(t), C.byref(p), C.byref(ms))
         return t.value, p.value, ms.value

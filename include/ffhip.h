@@ -290,9 +290,10 @@ int ffhip_ensemble_destroy(ffhip_ensemble* e);
 /* nsteps ETDRK4 steps of every member; sols[m] is member m's rfft state on
  * the device (nkr x ny, state precision), updated in place. */
 int ffhip_ensemble_run(ffhip_ensemble* e, void* const* sols, int64_t nsteps, void* stream);
-/* the variant in use: threads per workgroup, points-per-thread rule
- * (0: 8 points everywhere, 1: 16 points from n = 256 up) and, if tuned, its
- * measured ms per ensemble step. */
+/* the variant in use: threads per workgroup, the points per thread of every
+ * grid size and pass -- *prule bit s (n = 64 << s) set: 16 points per thread
+ * in the column passes of size-n members, else 8; bit 8 + s the same for the
+ * row passes -- and, if tuned, its measured ms per ensemble step. */
 int ffhip_ensemble_info(const ffhip_ensemble* e, int* threads, int* prule, double* ms_per_step);
 
 /* ---------------------------------------------------------------- profiling

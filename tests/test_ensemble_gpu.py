@@ -106,6 +106,30 @@ def test_ensemble_fp64_matches_oracle(ff):
         assert err < 1e-12, (m, err)
 
 
+@pytest.mark.parametrize("variant", ["128,2048,2048", "256,128,128", "256,512,128", "128,128,512", "256,256,2048"])
+def test_fixed_variants(ff, variant, monkeypatch):
+    """Every (grid size, pass) body with 8 and with 16 points per thread, in
+    both workgroup sizes, equals the single-grid runs (FP64).  A variant is
+    "workgroup size, column threshold, row threshold": sizes from a threshold
+    up take 16 points per thread."""
+    dt, steps = 0.004, 6
+    seeds = list(range(51, 51 + len(MEMBERS)))
+    monkeypatch.setenv("FFHIP_ENS_VARIANT", variant)
+    ens = ff.VorticityEnsemble(MEMBERS, dt, T=np.float64)
+    monkeypatch.delenv("FFHIP_ENS_VARIANT")
+    bd, tc, tr = (int(x) for x in variant.split(","))
+    mask = lambda t: sum(1 << s for s in range(5) if (64 << s) >= t)  # noqa: E731
+    threads, masks, _ = ens.info()
+    assert (threads, masks) == (bd, mask(tc) | (mask(tr) << 8))
+    singles = _singles(ff, MEMBERS, dt, np.float64, seeds)
+    for s, p in zip(ens.sols, singles):
+        s.copy_(p.sol)
+    ens.run(steps)
+    for m, (s, p) in enumerate(zip(ens.sols, singles)):
+        ff.stepforward(p, steps)
+        assert ou.rel_l2(_host(s), _host(p.sol)) < 1e-12, (m, variant)
+
+
 @pytest.mark.parametrize("T,tol", [(np.float64, 1e-12), (np.float32, 2e-5)])
 def test_autotuned_ensemble(ff, T, tol):
     dt, steps = 0.004, 20
@@ -116,7 +140,7 @@ def test_autotuned_ensemble(ff, T, tol):
         s.copy_(p.sol)
     ens.run(steps)
     threads, prule, ms = ens.info()
-    assert threads in (128, 256) and prule in (0, 1) and ms > 0
+    assert threads in (128, 256) and 0 <= prule < (1 << 13) and ms > 0
     for m, (s, p) in enumerate(zip(ens.sols, singles)):
         ff.stepforward(p, steps)
         a, b = _host(s).astype(np.complex128), _host(p.sol).astype(np.complex128)

@@ -28,91 +28,11 @@
 #include <new>
 #include <vector>
 
-#include "vort2d_body.h"
+#include "vort2d_ens.h"
 
 namespace ffhip {
 
-struct EnsMember {  // device-resident descriptor of one member
-  VortGeom g;
-  VortColArgs ca;
-  VortRowArgs ra;
-};
-
 constexpr int kEnsChunk = 10;  // steps per captured graph
-
-// points per thread of size n when sizes from t up take 16
-constexpr int ens_pts(int n, int t) { return n >= t ? 16 : 8; }
-constexpr bool ens_ok(int n, int t, int bd) { return n / ens_pts(n, t) <= bd; }
-constexpr int kEnsThresholds[4] = {128, 256, 512, 2048};  // 2048: 8 points at every ensemble size
-constexpr int kEnsDefaultThr = 256;                       // the single-grid rule
-// per-size mask of a threshold (bit s: size 64 << s takes 16 points)
-constexpr int ens_mask(int t) {
-  int m = 0;
-  for (int s = 0; s < 5; ++s)
-    if ((64 << s) >= t) m |= 1 << s;
-  return m;
-}
-
-template <typename R, int BD, int P, int N, int STAGE>
-__device__ __forceinline__ void ens_col_n(const VortGeom& g, const VortColArgs& a, int first, unsigned char* sm) {
-  constexpr int T = N / P;
-  if constexpr (T <= BD) {
-    constexpr int CPB = BD / T;
-    const int t = threadIdx.x;
-    const int cc = t % CPB;
-    const int col = first + cc;
-    constexpr int SL = team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), CPB);
-    cplx<R>* base = reinterpret_cast<cplx<R>*>(sm);
-    const cplx<R>* tw = team_tw_load<N, P, R>(base + CPB * SL, a.tw);
-    vcol_body<R, N, P, STAGE>(g, a, col / kVCB, col % kVCB, t / CPB, base + cc * SL, tw, col / kVCB < g.TX);
-  }
-}
-
-template <typename R, int BD, int P, int N>
-__device__ __forceinline__ void ens_row_n(const VortGeom& g, const VortRowArgs& a, int first, unsigned char* sm) {
-  constexpr int T = N / P;
-  if constexpr (T <= BD) {
-    constexpr int RPB = BD / T;
-    const int t = threadIdx.x;
-    const int r = t % RPB;
-    constexpr int SL = team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), RPB);
-    cplx<R>* base = reinterpret_cast<cplx<R>*>(sm);
-    const cplx<R>* tw = team_tw_load<N, P, R>(base + RPB * SL, a.tw);
-    vrow_body<R, N, P>(g, a, first + r, t / RPB, base + r * SL, tw);
-  }
-}
-
-// The member's size picks the body (uniform over the workgroup: every item
-// is one member); TH (threshold) fixes each size's points per thread.
-template <typename R, int BD, int TH, int STAGE>
-__global__ void __launch_bounds__(BD) k_ens_col(const EnsMember* __restrict__ mem, const int2* __restrict__ items) {
-  extern __shared__ unsigned char vsm[];
-  const int2 it = items[blockIdx.x];
-  const VortGeom g = mem[it.x].g;
-  const VortColArgs a = mem[it.x].ca;
-  switch (g.ny) {
-    case 64: ens_col_n<R, BD, ens_pts(64, TH), 64, STAGE>(g, a, it.y, vsm); break;
-    case 128: ens_col_n<R, BD, ens_pts(128, TH), 128, STAGE>(g, a, it.y, vsm); break;
-    case 256: ens_col_n<R, BD, ens_pts(256, TH), 256, STAGE>(g, a, it.y, vsm); break;
-    case 512: ens_col_n<R, BD, ens_pts(512, TH), 512, STAGE>(g, a, it.y, vsm); break;
-    default: ens_col_n<R, BD, ens_pts(1024, TH), 1024, STAGE>(g, a, it.y, vsm); break;
-  }
-}
-
-template <typename R, int BD, int TH>
-__global__ void __launch_bounds__(BD) k_ens_row(const EnsMember* __restrict__ mem, const int2* __restrict__ items) {
-  extern __shared__ unsigned char vsm[];
-  const int2 it = items[blockIdx.x];
-  const VortGeom g = mem[it.x].g;
-  const VortRowArgs a = mem[it.x].ra;
-  switch (g.nx) {
-    case 64: ens_row_n<R, BD, ens_pts(64, TH), 64>(g, a, it.y, vsm); break;
-    case 128: ens_row_n<R, BD, ens_pts(128, TH), 128>(g, a, it.y, vsm); break;
-    case 256: ens_row_n<R, BD, ens_pts(256, TH), 256>(g, a, it.y, vsm); break;
-    case 512: ens_row_n<R, BD, ens_pts(512, TH), 512>(g, a, it.y, vsm); break;
-    default: ens_row_n<R, BD, ens_pts(1024, TH), 1024>(g, a, it.y, vsm); break;
-  }
-}
 
 }  // namespace ffhip
 
@@ -121,6 +41,7 @@ using namespace ffhip;
 struct EnsVariant {
   int bd = 128;
   int tcol = kEnsDefaultThr, trow = kEnsDefaultThr;  // sizes from these up take 16 points per thread
+  bool pair = false;  // one launch per stage pair (tcol == trow, 256 or 2048)
 };
 
 struct ffhip_ensemble {
@@ -142,6 +63,8 @@ struct ffhip_ensemble {
   hipStream_t cap = nullptr;
   hipGraphExec_t exec = nullptr;
   int items_gen = 0, graph_gen = -1;  // the graph holds the item-table pointer it was captured with
+  void* sync = nullptr;    // stage-pair counters: rows_done[count], cols_done[count], err
+  void* counts = nullptr;  // int2[count]: (row items, column items) per member for var
   double bytes[6] = {0, 0, 0, 0, 0, 0};  // algorithmic bytes per pass over all members
 };
 
@@ -150,6 +73,7 @@ namespace ffhip {
 static bool pow2i(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 static bool variant_ok(const ffhip_ensemble* e, EnsVariant v) {
+  if (v.pair && (v.tcol != v.trow || (v.tcol != 256 && v.tcol != 2048))) return false;
   for (const auto& g : e->geo)
     if (!ens_ok(g.nx, v.trow, v.bd) || !ens_ok(g.ny, v.tcol, v.bd)) return false;
   return true;
@@ -163,6 +87,7 @@ static int build_items(ffhip_ensemble* e, EnsVariant v) {
     return (int64_t)e->geo[a].nx * e->geo[a].ny > (int64_t)e->geo[b].nx * e->geo[b].ny;
   });
   std::vector<int2> cols, rows;
+  std::vector<int2> cnt(e->count, make_int2(0, 0));
   size_t lc = 0, lr = 0;
   const size_t cs = cplx_size(e->prec);
   for (int m : order) {
@@ -173,6 +98,7 @@ static int build_items(ffhip_ensemble* e, EnsVariant v) {
              v.tcol, v.trow, g.nx, g.ny);
     for (int c = 0; c < g.TX * kVCB; c += cpb) cols.push_back(make_int2(m, c));
     for (int y = 0; y < g.ny; y += rpb) rows.push_back(make_int2(m, y));
+    cnt[m] = make_int2((g.ny + rpb - 1) / rpb, (g.TX * kVCB + cpb - 1) / cpb);
     lc = std::max(lc, ((size_t)cpb * (size_t)team_slice(g.ny + g.ny / 16, (int)cs, cpb) + tw_slots_rt(g.ny, Py)) * cs);
     lr = std::max(lr, ((size_t)rpb * (size_t)team_slice(g.nx + g.nx / 16, (int)cs, rpb) + tw_slots_rt(g.nx, Px)) * cs);
   }
@@ -184,6 +110,8 @@ static int build_items(ffhip_ensemble* e, EnsVariant v) {
   all.insert(all.end(), rows.begin(), rows.end());
   FF_TRY(dev_alloc(&e->items, all.size() * sizeof(int2)));
   FF_HIP(hipMemcpy(e->items, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice));
+  if (!e->counts) FF_TRY(dev_alloc(&e->counts, (size_t)e->count * sizeof(int2)));
+  FF_HIP(hipMemcpy(e->counts, cnt.data(), cnt.size() * sizeof(int2), hipMemcpyHostToDevice));
   e->ncol = (int)cols.size();
   e->nrow = (int)rows.size();
   e->lds_col = lc;
@@ -262,12 +190,37 @@ static int ens_launch(ffhip_ensemble* e, int which, hipStream_t st, bool attr_on
                               : ens_pass_v<double>(e, which, st, attr_only);
 }
 
+static EnsPairArgs pair_args(const ffhip_ensemble* e, int stage) {
+  EnsPairArgs pa;
+  pa.mem = reinterpret_cast<const EnsMember*>(e->dmem);
+  pa.items = reinterpret_cast<const int2*>(e->items);
+  pa.ncol = e->ncol;
+  pa.nrow = e->nrow;
+  int* sy = reinterpret_cast<int*>(e->sync);
+  pa.sy.rows_done = sy;
+  pa.sy.cols_done = sy + e->count;
+  pa.sy.err = sy + 2 * e->count;
+  pa.sy.counts = reinterpret_cast<const int2*>(e->counts);
+  pa.lds = std::max(e->lds_row, e->lds_col);
+  pa.bytes = e->bytes[5] + e->bytes[stage];
+  return pa;
+}
+
 static int ens_step(ffhip_ensemble* e, hipStream_t st) {
   for (int stage = 1; stage <= 4; ++stage) {
-    FF_TRY(ens_launch(e, 5, st));
-    FF_TRY(ens_launch(e, stage, st));
+    if (e->var.pair) {
+      FF_TRY(ens_pair_launch(e->prec, e->var.bd, e->var.tcol, stage, pair_args(e, stage), st, false));
+    } else {
+      FF_TRY(ens_launch(e, 5, st));
+      FF_TRY(ens_launch(e, stage, st));
+    }
   }
   return FFHIP_OK;
+}
+
+// the stage-pair counters start at zero (a fill kernel, outside any capture)
+static __global__ void k_zero_ints(int* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 static int ens_import(ffhip_ensemble* e, void* const* sols, hipStream_t st) {
@@ -307,6 +260,9 @@ static int capture_graph(ffhip_ensemble* e) {
 // LDS limits of every pass of the current variant
 static int set_variant_attrs(ffhip_ensemble* e) {
   for (int w = 0; w <= 5; ++w) FF_TRY(ens_launch(e, w, nullptr, true));
+  if (e->var.pair)
+    for (int stage = 1; stage <= 4; ++stage)
+      FF_TRY(ens_pair_launch(e->prec, e->var.bd, e->var.tcol, stage, pair_args(e, stage), nullptr, true));
   return FFHIP_OK;
 }
 
@@ -318,6 +274,11 @@ static int time_variant(ffhip_ensemble* e, EnsVariant v, hipStream_t st, hipEven
   FF_HIP(hipStreamSynchronize(st));  // the item table is replaced below
   FF_TRY(build_items(e, v));
   FF_TRY(set_variant_attrs(e));
+  int* sync = reinterpret_cast<int*>(e->sync);
+  if (v.pair) {
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(256), 0, st, sync, 2 * e->count + 1);
+    FF_LAUNCHED("k_zero_ints");
+  }
   for (int k = 0; k < 2; ++k) FF_TRY(ens_step(e, st));
   FF_HIP(hipEventRecord(a, st));
   for (int k = 0; k < 5; ++k) FF_TRY(ens_step(e, st));
@@ -326,11 +287,21 @@ static int time_variant(ffhip_ensemble* e, EnsVariant v, hipStream_t st, hipEven
   float t = 0;
   FF_HIP(hipEventElapsedTime(&t, a, b));
   *ms = t / 5.0;
+  if (v.pair) {  // a variant whose waits ran out is not a candidate
+    int err = 0;
+    FF_HIP(hipMemcpy(&err, sync + 2 * e->count, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) *ms = -1;
+  }
+  static const bool log = getenv("FFHIP_ENS_TUNE_LOG") && atoi(getenv("FFHIP_ENS_TUNE_LOG")) != 0;
+  if (log)
+    fprintf(stderr, "ensemble variant bd=%d tcol=%d trow=%d pair=%d: %.4f ms/step\n", v.bd, v.tcol, v.trow,
+            (int)v.pair, *ms);
   return FFHIP_OK;
 }
 
-// Every (BD, column threshold, row threshold) that fits the members; the
-// fastest is kept.
+// Every (BD, column threshold, row threshold) that fits the members, with one
+// launch per pass, and every (BD, threshold) with one launch per stage pair;
+// the fastest is kept.
 static int autotune_search(ffhip_ensemble* e, hipStream_t st, hipEvent_t a, hipEvent_t b) {
   double best = 1e300;
   EnsVariant bestv = e->var;
@@ -348,6 +319,19 @@ static int autotune_search(ffhip_ensemble* e, hipStream_t st, hipEvent_t a, hipE
           bestv = v;
         }
       }
+  for (int bd : {128, 256})  // one launch per stage pair
+    for (int t : {256, 2048}) {
+      EnsVariant v;
+      v.bd = bd;
+      v.tcol = v.trow = t;
+      v.pair = true;
+      double ms = 0;
+      FF_TRY(time_variant(e, v, st, a, b, &ms));
+      if (ms >= 0 && ms < best) {
+        best = ms;
+        bestv = v;
+      }
+    }
   FF_CHECK(best < 1e300, FFHIP_ERR_UNSUPPORTED, "autotune: no variant fits the ensemble");
   FF_HIP(hipStreamSynchronize(st));
   FF_TRY(build_items(e, bestv));
@@ -376,6 +360,8 @@ static void ens_free(ffhip_ensemble* e) {
   dev_free(e->pool);
   dev_free(e->dmem);
   dev_free(e->items);
+  dev_free(e->sync);
+  dev_free(e->counts);
   for (void* p : e->Efull) dev_free(p);
   for (auto& kv : e->tw) dev_free(kv.second);
   delete e;
@@ -519,6 +505,7 @@ int ffhip_ensemble_create(ffhip_ensemble** out, int prec, int count, const int64
     }
     ffhip_etdrk4_destroy(ts);
   }
+  if (rc == FFHIP_OK) rc = dev_alloc(&e->sync, (2 * (size_t)count + 1) * sizeof(int));  // zero-filled
   if (rc == FFHIP_OK) rc = dev_alloc(&e->dmem, e->hmem.size() * sizeof(EnsMember));
   if (rc == FFHIP_OK) {
     hipError_t he = hipMemcpy(e->dmem, e->hmem.data(), e->hmem.size() * sizeof(EnsMember), hipMemcpyHostToDevice);
@@ -527,12 +514,15 @@ int ffhip_ensemble_create(ffhip_ensemble** out, int prec, int count, const int64
   if (rc == FFHIP_OK) {
     EnsVariant v;  // the single-grid kernels' radix schedule
     if (!variant_ok(e, v)) v.bd = 256;
-    // FFHIP_ENS_VARIANT="bd,tcol,trow" (thresholds 128, 256, 512 or 2048) fixes the untuned variant
+    // FFHIP_ENS_VARIANT="bd,tcol,trow[,pair]" (thresholds 128, 256, 512 or 2048; pair 0 or 1)
+    // fixes the untuned variant
     if (const char* ev = getenv("FFHIP_ENS_VARIANT")) {
       EnsVariant f;
+      int pair = 0;
       auto thr_ok = [](int t) { return t == 128 || t == 256 || t == 512 || t == 2048; };
-      if (sscanf(ev, "%d,%d,%d", &f.bd, &f.tcol, &f.trow) != 3 || (f.bd != 128 && f.bd != 256) ||
-          !thr_ok(f.tcol) || !thr_ok(f.trow) || !variant_ok(e, f))
+      const int nf = sscanf(ev, "%d,%d,%d,%d", &f.bd, &f.tcol, &f.trow, &pair);
+      f.pair = pair != 0;
+      if (nf < 3 || (f.bd != 128 && f.bd != 256) || !thr_ok(f.tcol) || !thr_ok(f.trow) || !variant_ok(e, f))
         rc = fail(FFHIP_ERR_ARG, "FFHIP_ENS_VARIANT=%s is not a valid variant for this ensemble", ev);
       v = f;
     }
@@ -561,6 +551,10 @@ int ffhip_ensemble_run(ffhip_ensemble* e, void* const* sols, int64_t nsteps, voi
   hipStream_t st = as_stream(stream);
   if (e->autotune && !e->tuned) FF_TRY(autotune(e, st));
   FF_TRY(set_variant_attrs(e));
+  if (e->var.pair) {
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(256), 0, st, reinterpret_cast<int*>(e->sync), 2 * e->count + 1);
+    FF_LAUNCHED("k_zero_ints");
+  }
   FF_TRY(ens_import(e, sols, st));
   FF_TRY(ens_launch(e, 0, st));
   int64_t left = nsteps;
@@ -573,6 +567,13 @@ int ffhip_ensemble_run(ffhip_ensemble* e, void* const* sols, int64_t nsteps, voi
   for (int m = 0; m < e->count; ++m)
     FF_TRY(vort_export(e->prec, e->geo[m], sols[m], e->hmem[m].ca.U, reinterpret_cast<const double*>(e->Efull[m]),
                        nullptr, nsteps, st));
+  if (e->var.pair) {  // did every column workgroup's wait end normally?
+    int err = 0;
+    FF_HIP(hipMemcpyAsync(&err, reinterpret_cast<int*>(e->sync) + 2 * e->count, sizeof(int), hipMemcpyDeviceToHost,
+                          st));
+    FF_HIP(hipStreamSynchronize(st));
+    FF_CHECK(err == 0, FFHIP_ERR_HIP, "ensemble stage-pair launch: a column workgroup's wait for its rows ran out");
+  }
   return FFHIP_OK;
 }
 
@@ -580,7 +581,7 @@ int ffhip_ensemble_info(const ffhip_ensemble* e, int* threads, int* prule, doubl
   clear_error();
   FF_CHECK(e, FFHIP_ERR_ARG, "null ensemble");
   if (threads) *threads = e->var.bd;
-  if (prule) *prule = ens_mask(e->var.tcol) | (ens_mask(e->var.trow) << 8);
+  if (prule) *prule = ens_mask(e->var.tcol) | (ens_mask(e->var.trow) << 8) | (e->var.pair ? 1 << 16 : 0);
   if (ms_per_step) *ms_per_step = e->tuned ? e->tuned_ms : 0.0;
   return FFHIP_OK;
 }

@@ -106,21 +106,23 @@ def test_ensemble_fp64_matches_oracle(ff):
         assert err < 1e-12, (m, err)
 
 
-@pytest.mark.parametrize("variant", ["128,2048,2048", "256,128,128", "256,512,128", "128,128,512", "256,256,2048"])
+@pytest.mark.parametrize("variant", ["128,2048,2048", "256,128,128", "256,512,128", "128,128,512", "256,256,2048",
+                                     "128,256,256,1", "256,2048,2048,1"])
 def test_fixed_variants(ff, variant, monkeypatch):
     """Every (grid size, pass) body with 8 and with 16 points per thread, in
-    both workgroup sizes, equals the single-grid runs (FP64).  A variant is
-    "workgroup size, column threshold, row threshold": sizes from a threshold
-    up take 16 points per thread."""
+    both workgroup sizes, one launch per pass or per stage pair, equals the
+    single-grid runs (FP64).  A variant is "workgroup size, column threshold,
+    row threshold[, stage pair]": sizes from a threshold up take 16 points
+    per thread."""
     dt, steps = 0.004, 6
     seeds = list(range(51, 51 + len(MEMBERS)))
     monkeypatch.setenv("FFHIP_ENS_VARIANT", variant)
     ens = ff.VorticityEnsemble(MEMBERS, dt, T=np.float64)
     monkeypatch.delenv("FFHIP_ENS_VARIANT")
-    bd, tc, tr = (int(x) for x in variant.split(","))
+    bd, tc, tr, pair = ([int(x) for x in variant.split(",")] + [0])[:4]
     mask = lambda t: sum(1 << s for s in range(5) if (64 << s) >= t)  # noqa: E731
     threads, masks, _ = ens.info()
-    assert (threads, masks) == (bd, mask(tc) | (mask(tr) << 8))
+    assert (threads, masks) == (bd, mask(tc) | (mask(tr) << 8) | (pair << 16))
     singles = _singles(ff, MEMBERS, dt, np.float64, seeds)
     for s, p in zip(ens.sols, singles):
         s.copy_(p.sol)
@@ -128,6 +130,27 @@ def test_fixed_variants(ff, variant, monkeypatch):
     for m, (s, p) in enumerate(zip(ens.sols, singles)):
         ff.stepforward(p, steps)
         assert ou.rel_l2(_host(s), _host(p.sol)) < 1e-12, (m, variant)
+
+
+@pytest.mark.parametrize("T", [np.float32, np.float64])
+def test_stage_pair_equals_per_pass(ff, T, monkeypatch):
+    """One launch per stage pair (per-member waits inside the launch) gives
+    the same bits as one launch per pass with the same radix schedule, over
+    graph replays (25 steps) and again in a second run."""
+    dt = 0.004
+    seeds = list(range(61, 61 + len(MEMBERS)))
+    outs = []
+    for variant in ("256,256,256,0", "256,256,256,1"):
+        monkeypatch.setenv("FFHIP_ENS_VARIANT", variant)
+        ens = ff.VorticityEnsemble(MEMBERS, dt, T=T)
+        monkeypatch.delenv("FFHIP_ENS_VARIANT")
+        for s, p in zip(ens.sols, _singles(ff, MEMBERS, dt, T, seeds)):
+            s.copy_(p.sol)
+        ens.run(25)
+        ens.run(13)
+        outs.append([_host(s) for s in ens.sols])
+    for m, (a, b) in enumerate(zip(*outs)):
+        assert np.array_equal(a, b), m
 
 
 @pytest.mark.parametrize("T,tol", [(np.float64, 1e-12), (np.float32, 2e-5)])
@@ -140,7 +163,7 @@ def test_autotuned_ensemble(ff, T, tol):
         s.copy_(p.sol)
     ens.run(steps)
     threads, prule, ms = ens.info()
-    assert threads in (128, 256) and 0 <= prule < (1 << 13) and ms > 0
+    assert threads in (128, 256) and 0 <= prule < (1 << 17) and ms > 0
     for m, (s, p) in enumerate(zip(ens.sols, singles)):
         ff.stepforward(p, steps)
         a, b = _host(s).astype(np.complex128), _host(p.sol).astype(np.complex128)

@@ -32,7 +32,7 @@ sys.path.insert(0, os.path.join(ROOT, "fourierflows.jl_amd"))
 METRIC = "ETDRK4 steps/s at 2048² FP32 vs FP64 on GPU; FFT HBM GB/s; adjoint/forward time"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vorticity_2048_fp32_hbm_traffic.json")
-ROCPROF_NAMES = {"vort_row": "k_vrow<float, 2048, 16>",
+ROCPROF_NAMES = {"vort_row": "k_vrowh<2048>",
                  **{f"vort_col_stage{s}": f"k_vcol<float, 2048, 16, {s}>" for s in range(5)}}
 N = 2048
 DT = 1e-3

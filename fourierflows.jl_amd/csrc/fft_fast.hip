@@ -108,7 +108,10 @@ __global__ void __launch_bounds__(256) k_fc2r(const cplx<R>* __restrict__ in, R*
 }
 
 // ----------------------------------------------------------------- columns
-constexpr int fcol_pts(int n) { return n >= 1024 ? 16 : 8; }
+#ifndef FF_FCOL_P16
+#define FF_FCOL_P16 1024  // columns of at least this length take 16 points per thread, shorter ones 8
+#endif
+constexpr int fcol_pts(int n) { return n >= FF_FCOL_P16 ? 16 : 8; }
 template <typename R, int N>
 constexpr int fcol_cols() {  // columns per workgroup: >= 256 threads, >= 32 B per row segment
   constexpr int T = N / fcol_pts(N);

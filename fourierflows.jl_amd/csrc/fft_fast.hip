@@ -124,12 +124,16 @@ constexpr int fcol_sub() {  // workgroups whose column segments share one 128 B 
   return seg >= 128 ? 1 : 128 / seg;
 }
 
-// block b runs on XCD b % 8: sub-block s of line group q gets b = (q % 8) + 8 ((q / 8) NSUB + s)
-__device__ __forceinline__ void fxcd_split(int b, int nsub, int* group, int* sub) {
+// Block b runs on XCD b % 8.  Each XCD takes a contiguous range of gpx line
+// groups (group q on XCD q / gpx), its NSUB workgroups per group side by
+// side: neighbouring groups then share an L2 too, which matters when the row
+// pitch is not a multiple of 128 B (an rfft array's nx/2 + 1 columns) and a
+// group's rows straddle the lines of the next group.
+__device__ __forceinline__ void fxcd_split(int b, int nsub, int gpx, int* group, int* sub) {
   const int x = b % 8;
   const int slot = b / 8;
   *sub = slot % nsub;
-  *group = (slot / nsub) * 8 + x;
+  *group = x * gpx + slot / nsub;
 }
 
 // Transform (slab s, column c) has element i at s * slab_dist + c + i * stride;
@@ -138,14 +142,14 @@ __device__ __forceinline__ void fxcd_split(int b, int nsub, int* group, int* sub
 template <typename R, int N, int DIR>
 __global__ void __launch_bounds__((fcol_cols<R, N>() * (N / fcol_pts(N)))) k_fcol(
     const cplx<R>* in, cplx<R>* out, const cplx<R>* __restrict__ roots, int64_t ncols, int64_t stride,
-    int64_t slab_dist, R scale) {
+    int64_t slab_dist, R scale, int gpx) {
   extern __shared__ unsigned char fsm[];
   constexpr int P = fcol_pts(N);
   constexpr int T = N / P;
   constexpr int C = fcol_cols<R, N>();
   constexpr int SL = team_slice(TeamSched<N, P>::LDS, sizeof(cplx<R>), C);
   int grp, sub;
-  fxcd_split(blockIdx.x, fcol_sub<R, N>(), &grp, &sub);
+  fxcd_split(blockIdx.x, fcol_sub<R, N>(), gpx, &grp, &sub);
   const int64_t col0 = ((int64_t)grp * fcol_sub<R, N>() + sub) * C;
   if (col0 >= ncols) return;  // whole workgroup, before any barrier
   const int t = threadIdx.x;
@@ -262,12 +266,13 @@ static int cols_n_dir(const void* in, void* out, int64_t ncols, int64_t stride, 
   }
   FF_CHECK(nslabs <= 65535, FFHIP_ERR_UNSUPPORTED, "fast column pass: %lld slabs", (long long)nslabs);
   const int64_t groups = (ncols + (int64_t)C * SUB - 1) / ((int64_t)C * SUB);
-  const int64_t gx = (groups + 7) / 8 * 8 * SUB;
+  const int64_t gpx = (groups + 7) / 8;  // line groups per XCD
+  const int64_t gx = gpx * 8 * SUB;
   FF_CHECK(gx < (1LL << 31), FFHIP_ERR_UNSUPPORTED, "fast column pass: %lld columns", (long long)ncols);
   const double bytes = 2.0 * ncols * nslabs * N * sizeof(cplx<R>);
   prof_launch("fft_cols_c2c", bytes, k_fcol<R, N, DIR>, dim3((unsigned)gx, (unsigned)nslabs), dim3(C * (N / P)), sh,
               st, reinterpret_cast<const cplx<R>*>(in), reinterpret_cast<cplx<R>*>(out),
-              reinterpret_cast<const cplx<R>*>(ax.tw), ncols, stride, slab_dist, (R)scale);
+              reinterpret_cast<const cplx<R>*>(ax.tw), ncols, stride, slab_dist, (R)scale, (int)gpx);
   FF_LAUNCHED("k_fcol");
   return FFHIP_OK;
 }

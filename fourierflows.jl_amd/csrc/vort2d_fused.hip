@@ -185,12 +185,14 @@ static int launch_col(const VortGeom& g, const VortColArgs& a, hipStream_t st) {
   return FFHIP_OK;
 }
 
-// Three-team passes (measured on MI355X at 2048^2: faster in FP64, slower in
-// FP32, where the single-team kernels keep two waves per SIMD busy).
-// FFHIP_COL3 / FFHIP_ROW3: 0 off, 1 FP64 only (default), 2 FP32 and FP64.
+// Three-team passes (two transforms side by side per column or row pair).
+// Measured on MI355X at 2048^2 with the register-twiddle single-team
+// kernels: slower in FP32 (the single-team kernels keep two waves per SIMD
+// busy) and in FP64 (1953 vs 1891 steps/s with both off), so off by default.
+// FFHIP_COL3 / FFHIP_ROW3: 0 off (default), 1 FP64 only, 2 FP32 and FP64.
 static bool use3(const char* var, bool f32) {
   const char* e = getenv(var);
-  const int mode = e ? atoi(e) : 1;
+  const int mode = e ? atoi(e) : 0;
   return f32 ? mode >= 2 : mode >= 1;
 }
 // FFHIP_ROWH=0 selects the two-row team kernel for FP32 rows where the
@@ -202,14 +204,10 @@ static bool rowh_enabled() {
   }();
   return on;
 }
-static bool col3_on(bool f32) {
-  static const bool on32 = use3("FFHIP_COL3", true), on64 = use3("FFHIP_COL3", false);
-  return f32 ? on32 : on64;
-}
-static bool row3_on(bool f32) {
-  static const bool on32 = use3("FFHIP_ROW3", true), on64 = use3("FFHIP_ROW3", false);
-  return f32 ? on32 : on64;
-}
+// read at every launch (eager steps and graph capture), so a switch set
+// before a problem's first run applies to that problem
+static bool col3_on(bool f32) { return use3("FFHIP_COL3", f32); }
+static bool row3_on(bool f32) { return use3("FFHIP_ROW3", f32); }
 
 template <typename R, int N, int STAGE>
 static int launch_col3(const VortGeom& g, const VortColArgs& a, hipStream_t st) {

@@ -54,7 +54,7 @@ def test_fused_fp64_matches_oracle(ff, n, ny, Ly):
     o = og.TwoDGrid(n, 2 * np.pi, ny, Ly, x0=0, y0=0)
     zh0 = oics.vorticity_ic(o, seed=11, k0=min(6.0, n / 10))
     L = oph.vorticity_linear(o, nu=2e-3)
-    steps = 30 if n * ny <= 512 * 512 else 6  # >= 1024: the three-team FP64 passes
+    steps = 30 if n * ny <= 512 * 512 else 6
     ref = _oracle(n, ny, Ly, zh0, L, 0.004, steps)
     prob = ff.TwoDVorticityProblem(nx=n, ny=ny, Ly=Ly, nu=2e-3, dt=0.004)
     assert prob.eqn.calcN.fused
@@ -82,6 +82,36 @@ def test_fused_fp32_matches_oracle(ff, n, ny, Ly):
     ff.stepforward(prob, steps)
     err = ou.rel_l2(_host(prob.sol).astype(np.complex128), ref)
     assert err < 1e-5, err
+
+
+_THREE_TEAM_REF = {}
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("n,ny,Ly", [(1024, 1024, 2 * np.pi), (2048, 1024, np.pi), (1024, 2048, 2 * np.pi)])
+def test_fused_three_team_passes(ff, monkeypatch, prec, n, ny, Ly):
+    """The three-team row and column passes (FFHIP_COL3 / FFHIP_ROW3 = 2:
+    both precisions, 1024- and 2048-point axes), off by default, against
+    the FP64 oracle."""
+    T = np.float64 if prec == 64 else np.float32
+    CT = np.complex128 if prec == 64 else np.complex64
+    key = (n, ny, Ly)
+    if key not in _THREE_TEAM_REF:
+        o = og.TwoDGrid(n, 2 * np.pi, ny, Ly, x0=0, y0=0)
+        zh0 = oics.vorticity_ic(o, seed=13, k0=6.0)
+        L = oph.vorticity_linear(o, nu=2e-3)
+        _THREE_TEAM_REF[key] = (zh0, _oracle(n, ny, Ly, zh0, L, 0.004, 12))
+    zh0, ref = _THREE_TEAM_REF[key]
+    monkeypatch.setenv("FFHIP_COL3", "2")
+    monkeypatch.setenv("FFHIP_ROW3", "2")
+    prob = ff.TwoDVorticityProblem(nx=n, ny=ny, Ly=Ly, nu=2e-3, dt=0.004, T=T)
+    assert prob.eqn.calcN.fused
+    prob.sol.copy_(_dev(zh0.astype(CT)))
+    ff.stepforward(prob, 12)  # an eager step, one 10-step graph replay, an eager last step
+    err = ou.rel_l2(_host(prob.sol).astype(np.complex128), ref)
+    monkeypatch.delenv("FFHIP_COL3")
+    monkeypatch.delenv("FFHIP_ROW3")
+    assert err < (1e-12 if prec == 64 else 1e-5), err
 
 
 def test_fused_fp32_4096_square(ff):

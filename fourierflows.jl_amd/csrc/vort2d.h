@@ -22,6 +22,9 @@ namespace ffhip {
 #ifndef FF_ROW_PREFETCH_F64
 #define FF_ROW_PREFETCH_F64 1
 #endif
+#ifndef FF_ROWH_UVPRE
+#define FF_ROWH_UVPRE 1  // wavefront row pass: u^, v^ loaded before the zeta transform
+#endif
 #ifndef FF_STAGE_UNROLL
 #define FF_STAGE_UNROLL 16  // stage-combination loads in flight per thread
 #endif

@@ -425,6 +425,35 @@ __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs&
   const int KX = g.KX;
   TwRegs<M, Q, R> twr;
   twr_fill_roots_half<M, Q, 1, R, TwRegs<M, Q, R>::NR>(twr, roots, lane);
+#if FF_ROWH_UVPRE
+  // The row's zeta^, u^ and v^ (k < KXp) are loaded in 16 B chunks
+  // c = lane + 64 i holding k = 2c, 2c + 1, each element once, zeta^ first;
+  // u^ and v^ stay in registers across the zeta transform, which hides their
+  // load behind it.  They reach the lanes that combine k with M - k through
+  // LDS.
+  constexpr int NCH = ((NX / 3 + 4) / 2 + 63) / 64;  // KXp <= NX/3 + 4
+  using f4 = float __attribute__((ext_vector_type(4)));  // a native vector: stays in registers
+  const int nch = g.KXp >> 1;
+  f4 rz[NCH], ru[NCH], rv[NCH];
+  cplx<R> omz[Q];  // the roots the zeta combination needs, loaded ahead of u^, v^ (loads retire in order)
+#pragma unroll
+  for (int q = 0; q < Q; ++q) omz[q] = roots[lane + 64 * q];
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) {
+    const int c = min(lane + 64 * i, nch - 1);  // the tail lanes repeat the last chunk
+    rz[i] = at32(reinterpret_cast<const f4*>(Zh), iaddr(2 * c) >> 1);
+  }
+  __builtin_amdgcn_sched_barrier(0);  // issue order: zeta^ before u^, v^
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) {
+    const int c = min(lane + 64 * i, nch - 1);
+    ru[i] = at32(reinterpret_cast<const f4*>(Uh), iaddr(2 * c) >> 1);
+    rv[i] = at32(reinterpret_cast<const f4*>(Vh), iaddr(2 * c) >> 1);
+  }
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) reinterpret_cast<f4*>(lds1)[min(lane + 64 * i, nch - 1)] = rz[i];
+  team_sync<true>();
+#endif
 
   // zeta (2n) + i zeta (2n + 1).  Branch-free loads: out-of-band modes read
   // column 0 and are discarded; c2r ignores Im at kx = 0 and the Nyquist
@@ -434,24 +463,46 @@ __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs&
   for (int q = 0; q < Q; ++q) {
     const int k = lane + 64 * q, m = M - k;
     const bool okk = k < KX, okm = m < KX;
+#if FF_ROWH_UVPRE
+    const cplx<R> Xk = lds1[okk ? k : 0], Xm = lds1[okm ? m : 0];
+    const cplx<R> om = omz[q];
+#else
     const cplx<R> Xk = at32(Zh, iaddr(okk ? k : 0)), Xm = at32(Zh, iaddr(okm ? m : 0));
     const cplx<R> om = roots[k];
+#endif
     const R xkx = okk ? Xk.x : R(0), xky = (okk && k != 0) ? Xk.y : R(0);
     const R xmx = okm ? Xm.x : R(0), xmy = okm ? Xm.y : R(0);
     const R dx = xkx - xmx, dy = xky + xmy;                           // X[k] - conj(X[M-k])
     const R aox = dx * om.x + dy * om.y, aoy = dy * om.x - dx * om.y;  // times w^k = conj(root k)
     zr[q] = mkc<R>(xkx + xmx - aoy, xky - xmy + aox);                 // Ae + i Ao
   }
+#if FF_ROWH_UVPRE
+  team_sync<true>();  // every lane's reads of zeta^ before the transform overwrites them
+#endif
   wave_fft<M, Q, 1, R>(zr, lds1, lane, twr);
 
   // u + i v at the even (xa) and odd (xb) samples
+#if FF_ROWH_UVPRE
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) {  // u^ into lds1, v^ into lds2 (free after the transform)
+    const int c = min(lane + 64 * i, nch - 1);  // tail lanes rewrite the last chunk with its own value
+    reinterpret_cast<f4*>(lds1)[c] = ru[i];
+    reinterpret_cast<f4*>(lds2)[c] = rv[i];
+  }
+  team_sync<true>();
+#endif
   cplx<R> xa[Q], xb[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int k = lane + 64 * q, m = M - k;
     const bool okk = k < KX, okm = m < KX;
+#if FF_ROWH_UVPRE
+    const int ik = okk ? k : 0, im = okm ? m : 0;
+    const cplx<R> U = lds1[ik], V = lds2[ik], Um = lds1[im], Vm = lds2[im];
+#else
     const uint32_t ik = iaddr(okk ? k : 0), im = iaddr(okm ? m : 0);
     const cplx<R> U = at32(Uh, ik), V = at32(Vh, ik), Um = at32(Uh, im), Vm = at32(Vh, im);
+#endif
     const cplx<R> om = roots[k];
     const R uy = k == 0 ? R(0) : U.y, vy = k == 0 ? R(0) : V.y;
     const R slx = okk ? U.x - vy : R(0), sly = okk ? uy + V.x : R(0);       // S[k] = U + i V
@@ -460,6 +511,9 @@ __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs&
     const R dx = slx - shx, dy = sly - shy;
     xb[q] = mkc<R>(dx * om.x + dy * om.y, dy * om.x - dx * om.y);
   }
+#if FF_ROWH_UVPRE
+  team_sync<true>();  // every lane's reads of u^, v^ before the transform overwrites them
+#endif
   wave_fft2<M, Q, 1, R>(xa, xb, lds1, lds2, lane, twr);
 
   // products u zeta + i v zeta with the 1/(nx ny)^2 normalisation

@@ -258,6 +258,9 @@ static int launch_rowh(const VortGeom& g, const VortRowArgs& a, hipStream_t st) 
     attr_set = true;
   }
   FF_CHECK(g.ny % 4 == 0, FFHIP_ERR_UNSUPPORTED, "wavefront row kernel: ny = %d", g.ny);
+  // the kept band's 16 B chunks fit the lanes' prefetch registers (KXp <= N/3 + 4)
+  FF_CHECK(g.KXp <= 128 * (((N / 3 + 4) / 2 + 63) / 64), FFHIP_ERR_UNSUPPORTED, "wavefront row kernel: KXp = %d",
+           g.KXp);
   prof_launch("vort_row", row_bytes<float>(g), kern, dim3(g.ny / 4), dim3(256), sh, st, g, a);
   FF_LAUNCHED("k_vrowh");
   return FFHIP_OK;

@@ -19,11 +19,20 @@ namespace ffhip {
 #ifndef FF_COL_TWREG
 #define FF_COL_TWREG 1  // FP32 column pass: first-pass twiddles in registers across its five transforms
 #endif
+#ifndef FF_ROW_TWREG_F64
+#define FF_ROW_TWREG_F64 0  // the same for FP64 rows (measured no faster at 2048^2)
+#endif
+#ifndef FF_COL_TWREG_F64
+#define FF_COL_TWREG_F64 1  // FP64 column pass: first-pass twiddles in registers (2048^2: stages ~1% faster)
+#endif
 #ifndef FF_ROW_PREFETCH_F64
 #define FF_ROW_PREFETCH_F64 1
 #endif
 #ifndef FF_ROWH_UVPRE
 #define FF_ROWH_UVPRE 1  // wavefront row pass: u^, v^ loaded before the zeta transform
+#endif
+#ifndef FF_COL_O1PRE
+#define FF_COL_O1PRE 1  // FP64 column pass: second forward input loaded before the first transform
 #endif
 #ifndef FF_STAGE_UNROLL
 #define FF_STAGE_UNROLL 16  // stage-combination loads in flight per thread
@@ -74,21 +83,30 @@ constexpr int vpts_row(int n) { return n >= 256 ? 16 : 8; }
 // (512 = 8^3): twice the threads per column for the same passes and fewer
 // registers (measured on MI355X: 512^2 forward 46.0 -> 37.4 ms per 500
 // steps).  Where 8 adds a pass (1024 and 2048: 4 vs 3) it stays at 16
-// (2048^2 measured 34.5 vs 27.5 us per stage).
+// (2048^2 measured 34.5 vs 27.5 us per stage).  FP64 at 2048 takes 8
+// although it adds a pass: 16 FP64 points per thread need more than 256
+// registers, one wave per SIMD, and 8 fit two (2048^2 FP64 measured: column
+// stages 65 -> 55 us, rows 59 -> 57 us, 1.96k -> 2.16k steps/s).
 #ifndef FF_COL_P8_ALWAYS
 #define FF_COL_P8_ALWAYS 0  // 1: FP32 columns take 8 points per thread at every size (measurement switch)
 #endif
+#ifndef FF_COL_P8_F64
+#define FF_COL_P8_F64 1  // FP64 columns of 2048 points take 8 points per thread (0: 16)
+#endif
+#ifndef FF_ROW_P8_F64
+#define FF_ROW_P8_F64 1  // FP64 rows of 2048 points take 8 points per thread (0: 16)
+#endif
 template <typename R, int N>
 constexpr int vcol_pts() {
-  return sizeof(R) == 4 && N >= 256 && (FF_COL_P8_ALWAYS || TeamSched<N, 8>::NPASS == TeamSched<N, 16>::NPASS)
-             ? 8
-             : vpts_col(N);
+  if constexpr (sizeof(R) == 8) return FF_COL_P8_F64 && N == 2048 ? 8 : vpts_col(N);
+  return N >= 256 && (FF_COL_P8_ALWAYS || TeamSched<N, 8>::NPASS == TeamSched<N, 16>::NPASS) ? 8 : vpts_col(N);
 }
 // The same rule for the single-team row passes, forward and adjoint (512^2
 // forward, measured: 37.3 -> 33.0 ms per 500 steps).
 template <typename R, int N>
 constexpr int vrow_pts() {
-  return sizeof(R) == 4 && N >= 256 && TeamSched<N, 8>::NPASS == TeamSched<N, 16>::NPASS ? 8 : vpts_row(N);
+  if constexpr (sizeof(R) == 8) return FF_ROW_P8_F64 && N == 2048 ? 8 : vpts_row(N);
+  return N >= 256 && TeamSched<N, 8>::NPASS == TeamSched<N, 16>::NPASS ? 8 : vpts_row(N);
 }
 
 struct VortGeom {

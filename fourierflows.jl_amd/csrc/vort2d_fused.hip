@@ -55,11 +55,17 @@ constexpr bool row_pair() {
 #ifndef FF_COL_WAVES
 #define FF_COL_WAVES 1  // same for the single-team column pass
 #endif
+#ifndef FF_ROW_WAVES_F64
+#define FF_ROW_WAVES_F64 1  // the same minimums for FP64
+#endif
+#ifndef FF_COL_WAVES_F64
+#define FF_COL_WAVES_F64 1
+#endif
 
 // STAGE 0: y-inverse of U only.  STAGE 1..4: ETDRK4 stages.  kCBK columns per
 // workgroup; the kCSUB workgroups of one 4-column tile run on the same XCD.
 template <typename R, int NY, int P, int STAGE>
-__global__ void __launch_bounds__(kCBK*(NY / P), sizeof(R) == 4 ? FF_COL_WAVES : 1) k_vcol(VortGeom g, VortColArgs a) {
+__global__ void __launch_bounds__(kCBK*(NY / P), sizeof(R) == 4 ? FF_COL_WAVES : FF_COL_WAVES_F64) k_vcol(VortGeom g, VortColArgs a) {
   extern __shared__ unsigned char vsm[];
   int tx, sub;
   xcd_split(blockIdx.x, kCSUB, &tx, &sub);
@@ -110,7 +116,7 @@ __global__ void __launch_bounds__(3 * (NX / P), sizeof(R) == 4 ? FF_ROW3_WAVES :
 
 // kRRK rows per workgroup; the kRSUB workgroups of one 4-row group run on the same XCD.
 template <typename R, int NX, int P>
-__global__ void __launch_bounds__(kRRK*(NX / P), sizeof(R) == 4 ? FF_ROW_WAVES : 1) k_vrow(VortGeom g, VortRowArgs a) {
+__global__ void __launch_bounds__(kRRK*(NX / P), sizeof(R) == 4 ? FF_ROW_WAVES : FF_ROW_WAVES_F64) k_vrow(VortGeom g, VortRowArgs a) {
   extern __shared__ unsigned char vsm[];
   int grp, sub;
   xcd_split(blockIdx.x, kRSUB, &grp, &sub);

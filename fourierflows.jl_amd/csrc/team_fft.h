@@ -123,27 +123,29 @@ static_assert(tw_slots<2048, 16>() == tw_slots_rt(2048, 16) && tw_slots<1024, 16
                   tw_slots<64, 8>() == tw_slots_rt(64, 8),
               "run-time twiddle table size must match the compile-time one");
 
-template <int N, int P, int S, typename R>
+// RS: roots holds the RS*N roots of unity (root j of N is entry RS*j)
+template <int N, int P, int S, typename R, int RS = 1>
 __device__ __forceinline__ void tw_fill_pass(cplx<R>* ltw, const cplx<R>* roots, int t, int nt) {
   using TS = TeamSched<N, P>;
   if constexpr (S < TS::NPASS) {
     constexpr int RK = S < TS::NFULL ? P : TS::REM;
     constexpr int NS = cpow(P, S);
-    constexpr int STEP = N / (NS * RK);
+    constexpr int STEP = RS * (N / (NS * RK));
     constexpr int OFF = tw_off<N, P, S>();
     for (int e = t; e < (RK - 1) * NS; e += nt) {
       const int r = e / NS + 1, k = e - (r - 1) * NS;
       ltw[OFF + e] = roots[r * k * STEP];
     }
-    tw_fill_pass<N, P, S + 1, R>(ltw, roots, t, nt);
+    tw_fill_pass<N, P, S + 1, R, RS>(ltw, roots, t, nt);
   }
 }
 
 // Fill the pass-ordered table from the N roots of unity (all threads of the
 // block take part; the block barrier at the end publishes it).  Returns ltw.
-template <int N, int P, typename R>
+// RS > 1 reads every RS-th entry of a table of RS*N roots.
+template <int N, int P, typename R, int RS = 1>
 __device__ __forceinline__ const cplx<R>* team_tw_load(cplx<R>* ltw, const void* roots) {
-  tw_fill_pass<N, P, 1, R>(ltw, reinterpret_cast<const cplx<R>*>(roots), (int)threadIdx.x, (int)blockDim.x);
+  tw_fill_pass<N, P, 1, R, RS>(ltw, reinterpret_cast<const cplx<R>*>(roots), (int)threadIdx.x, (int)blockDim.x);
   __syncthreads();
   return ltw;
 }

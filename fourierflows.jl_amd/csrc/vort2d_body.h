@@ -417,14 +417,24 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
 //   split:  P[k], Q[k] from W[k] and conj(W[NX-k]); W[NX-k] sits in lane
 //           (64 - l) mod 64, fetched with ds_bpermute (__shfl)
 // Five M-point transforms per row instead of three NX-point ones.  lds1 and
-// lds2 are this wavefront's two buffers of TeamSched<M, Q>::LDS slots; every
-// twiddle is in registers (filled from the NX roots of unity in tw).
-template <int NX, int Q>
+// lds2 are the row's two buffers of TeamSched<M, Q>::LDS slots.
+// Team size T = M/Q: 64 (FP32) is one wavefront, with every twiddle in
+// registers (filled from the NX roots of unity in tw) and wavefront-level
+// exchanges; a larger team (FP64: 128 threads, two wavefronts) takes its
+// factors from the LDS table ltw, exchanges through workgroup barriers (all
+// rows of the workgroup in step) and fetches the split's mirrored
+// coefficient from LDS instead of another lane.  In the text above read
+// "thread" for "lane" and T for 64 then.
+template <typename R, int NX, int Q>
 __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs& a, const int y, const int lane,
-                                           cplx<float>* lds1, cplx<float>* lds2) {
-  using R = float;
+                                           cplx<R>* lds1, cplx<R>* lds2, const cplx<R>* ltw = nullptr) {
   constexpr int M = NX / 2;
-  static_assert(M / Q == 64, "one wavefront per row");
+  constexpr int T = M / Q;
+  constexpr bool WAVE = T == 64;
+  static_assert(T % 64 == 0, "whole wavefronts per row");
+  static_assert(WAVE || sizeof(R) == 8, "FP32 rows run one wavefront each");
+  // zeta^, u^, v^ loaded up front in 16 B chunks (FP32 wavefront rows)
+  constexpr bool PRE = WAVE && FF_ROWH_UVPRE;
   constexpr int CB = kVCB;
   const uint32_t ny = (uint32_t)g.ny;
   auto iaddr = [&](int k) -> uint32_t { return ((uint32_t)(k / CB) * ny + (uint32_t)y) * CB + (k % CB); };
@@ -433,9 +443,23 @@ __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs&
   const cplx<R>* Uh = reinterpret_cast<const cplx<R>*>(a.I1);
   const cplx<R>* Vh = reinterpret_cast<const cplx<R>*>(a.I2);
   const int KX = g.KX;
-  TwRegs<M, Q, R> twr;
-  twr_fill_roots_half<M, Q, 1, R, TwRegs<M, Q, R>::NR>(twr, roots, lane);
-#if FF_ROWH_UVPRE
+  TwRegs<M, Q, R> twr;  // wavefront rows only
+  if constexpr (WAVE) twr_fill_roots_half<M, Q, 1, R, TwRegs<M, Q, R>::NR>(twr, roots, lane);
+  auto fft1 = [&](cplx<R>(&x)[Q]) {
+    if constexpr (WAVE)
+      wave_fft<M, Q, 1, R>(x, lds1, lane, twr);
+    else
+      team_fft<M, Q, 1, R>(x, lds1, lane, ltw);
+  };
+  auto fft2 = [&](cplx<R>(&x1)[Q], cplx<R>(&x2)[Q], auto dir) {
+    if constexpr (WAVE)
+      wave_fft2<M, Q, decltype(dir)::value, R>(x1, x2, lds1, lds2, lane, twr);
+    else
+      team_fft2<M, Q, decltype(dir)::value, R>(x1, x2, lds1, lds2, lane, ltw);
+  };
+  using Inv = std::integral_constant<int, 1>;
+  using Fwd = std::integral_constant<int, -1>;
+
   // The row's zeta^, u^ and v^ (k < KXp) are loaded in 16 B chunks
   // c = lane + 64 i holding k = 2c, 2c + 1, each element once, zeta^ first;
   // u^ and v^ stay in registers across the zeta transform, which hides their
@@ -446,24 +470,25 @@ __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs&
   const int nch = g.KXp >> 1;
   f4 rz[NCH], ru[NCH], rv[NCH];
   cplx<R> omz[Q];  // the roots the zeta combination needs, loaded ahead of u^, v^ (loads retire in order)
+  if constexpr (PRE) {
 #pragma unroll
-  for (int q = 0; q < Q; ++q) omz[q] = roots[lane + 64 * q];
+    for (int q = 0; q < Q; ++q) omz[q] = roots[lane + 64 * q];
 #pragma unroll
-  for (int i = 0; i < NCH; ++i) {
-    const int c = min(lane + 64 * i, nch - 1);  // the tail lanes repeat the last chunk
-    rz[i] = at32(reinterpret_cast<const f4*>(Zh), iaddr(2 * c) >> 1);
+    for (int i = 0; i < NCH; ++i) {
+      const int c = min(lane + 64 * i, nch - 1);  // the tail lanes repeat the last chunk
+      rz[i] = at32(reinterpret_cast<const f4*>(Zh), iaddr(2 * c) >> 1);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // issue order: zeta^ before u^, v^
+#pragma unroll
+    for (int i = 0; i < NCH; ++i) {
+      const int c = min(lane + 64 * i, nch - 1);
+      ru[i] = at32(reinterpret_cast<const f4*>(Uh), iaddr(2 * c) >> 1);
+      rv[i] = at32(reinterpret_cast<const f4*>(Vh), iaddr(2 * c) >> 1);
+    }
+#pragma unroll
+    for (int i = 0; i < NCH; ++i) reinterpret_cast<f4*>(lds1)[min(lane + 64 * i, nch - 1)] = rz[i];
+    team_sync<true>();
   }
-  __builtin_amdgcn_sched_barrier(0);  // issue order: zeta^ before u^, v^
-#pragma unroll
-  for (int i = 0; i < NCH; ++i) {
-    const int c = min(lane + 64 * i, nch - 1);
-    ru[i] = at32(reinterpret_cast<const f4*>(Uh), iaddr(2 * c) >> 1);
-    rv[i] = at32(reinterpret_cast<const f4*>(Vh), iaddr(2 * c) >> 1);
-  }
-#pragma unroll
-  for (int i = 0; i < NCH; ++i) reinterpret_cast<f4*>(lds1)[min(lane + 64 * i, nch - 1)] = rz[i];
-  team_sync<true>();
-#endif
 
   // zeta (2n) + i zeta (2n + 1).  Branch-free loads: out-of-band modes read
   // column 0 and are discarded; c2r ignores Im at kx = 0 and the Nyquist
@@ -471,48 +496,56 @@ __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs&
   cplx<R> zr[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int k = lane + 64 * q, m = M - k;
+    const int k = lane + T * q, m = M - k;
     const bool okk = k < KX, okm = m < KX;
-#if FF_ROWH_UVPRE
-    const cplx<R> Xk = lds1[okk ? k : 0], Xm = lds1[okm ? m : 0];
-    const cplx<R> om = omz[q];
-#else
-    const cplx<R> Xk = at32(Zh, iaddr(okk ? k : 0)), Xm = at32(Zh, iaddr(okm ? m : 0));
-    const cplx<R> om = roots[k];
-#endif
+    cplx<R> Xk, Xm, om;
+    if constexpr (PRE) {
+      Xk = lds1[okk ? k : 0];
+      Xm = lds1[okm ? m : 0];
+      om = omz[q];
+    } else {
+      Xk = at32(Zh, iaddr(okk ? k : 0));
+      Xm = at32(Zh, iaddr(okm ? m : 0));
+      om = roots[k];
+    }
     const R xkx = okk ? Xk.x : R(0), xky = (okk && k != 0) ? Xk.y : R(0);
     const R xmx = okm ? Xm.x : R(0), xmy = okm ? Xm.y : R(0);
     const R dx = xkx - xmx, dy = xky + xmy;                           // X[k] - conj(X[M-k])
     const R aox = dx * om.x + dy * om.y, aoy = dy * om.x - dx * om.y;  // times w^k = conj(root k)
     zr[q] = mkc<R>(xkx + xmx - aoy, xky - xmy + aox);                 // Ae + i Ao
   }
-#if FF_ROWH_UVPRE
-  team_sync<true>();  // every lane's reads of zeta^ before the transform overwrites them
-#endif
-  wave_fft<M, Q, 1, R>(zr, lds1, lane, twr);
+  if constexpr (PRE) team_sync<true>();  // every lane's reads of zeta^ before the transform overwrites them
+  fft1(zr);
 
   // u + i v at the even (xa) and odd (xb) samples
-#if FF_ROWH_UVPRE
+  if constexpr (PRE) {
 #pragma unroll
-  for (int i = 0; i < NCH; ++i) {  // u^ into lds1, v^ into lds2 (free after the transform)
-    const int c = min(lane + 64 * i, nch - 1);  // tail lanes rewrite the last chunk with its own value
-    reinterpret_cast<f4*>(lds1)[c] = ru[i];
-    reinterpret_cast<f4*>(lds2)[c] = rv[i];
+    for (int i = 0; i < NCH; ++i) {  // u^ into lds1, v^ into lds2 (free after the transform)
+      const int c = min(lane + 64 * i, nch - 1);  // tail lanes rewrite the last chunk with its own value
+      reinterpret_cast<f4*>(lds1)[c] = ru[i];
+      reinterpret_cast<f4*>(lds2)[c] = rv[i];
+    }
+    team_sync<true>();
   }
-  team_sync<true>();
-#endif
   cplx<R> xa[Q], xb[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int k = lane + 64 * q, m = M - k;
+    const int k = lane + T * q, m = M - k;
     const bool okk = k < KX, okm = m < KX;
-#if FF_ROWH_UVPRE
-    const int ik = okk ? k : 0, im = okm ? m : 0;
-    const cplx<R> U = lds1[ik], V = lds2[ik], Um = lds1[im], Vm = lds2[im];
-#else
-    const uint32_t ik = iaddr(okk ? k : 0), im = iaddr(okm ? m : 0);
-    const cplx<R> U = at32(Uh, ik), V = at32(Vh, ik), Um = at32(Uh, im), Vm = at32(Vh, im);
-#endif
+    cplx<R> U, V, Um, Vm;
+    if constexpr (PRE) {
+      const int ik = okk ? k : 0, im = okm ? m : 0;
+      U = lds1[ik];
+      V = lds2[ik];
+      Um = lds1[im];
+      Vm = lds2[im];
+    } else {
+      const uint32_t ik = iaddr(okk ? k : 0), im = iaddr(okm ? m : 0);
+      U = at32(Uh, ik);
+      V = at32(Vh, ik);
+      Um = at32(Uh, im);
+      Vm = at32(Vh, im);
+    }
     const cplx<R> om = roots[k];
     const R uy = k == 0 ? R(0) : U.y, vy = k == 0 ? R(0) : V.y;
     const R slx = okk ? U.x - vy : R(0), sly = okk ? uy + V.x : R(0);       // S[k] = U + i V
@@ -521,10 +554,8 @@ __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs&
     const R dx = slx - shx, dy = sly - shy;
     xb[q] = mkc<R>(dx * om.x + dy * om.y, dy * om.x - dx * om.y);
   }
-#if FF_ROWH_UVPRE
-  team_sync<true>();  // every lane's reads of u^, v^ before the transform overwrites them
-#endif
-  wave_fft2<M, Q, 1, R>(xa, xb, lds1, lds2, lane, twr);
+  if constexpr (PRE) team_sync<true>();  // every lane's reads of u^, v^ before the transform overwrites them
+  fft2(xa, xb, Inv{});
 
   // products u zeta + i v zeta with the 1/(nx ny)^2 normalisation
   const R s2 = (R)g.s2;
@@ -534,9 +565,16 @@ __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs&
     R* Fr = reinterpret_cast<R*>(reinterpret_cast<cplx<R>*>(a.F) + (int64_t)g.nx * g.ny) + (int64_t)y * NX;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int n = lane + 64 * q;
-      reinterpret_cast<float4*>(Fz)[n] = make_float4(xa[q].x * s1, xa[q].y * s1, xb[q].x * s1, xb[q].y * s1);
-      reinterpret_cast<float2*>(Fr)[n] = make_float2(zr[q].x * s1, zr[q].y * s1);
+      const int n = lane + T * q;  // samples 2n and 2n + 1
+      if constexpr (sizeof(R) == 4) {
+        reinterpret_cast<float4*>(Fz)[n] = make_float4(xa[q].x * s1, xa[q].y * s1, xb[q].x * s1, xb[q].y * s1);
+        reinterpret_cast<float2*>(Fr)[n] = make_float2(zr[q].x * s1, zr[q].y * s1);
+      } else {
+        Fz[2 * n] = mkc<R>(xa[q].x * s1, xa[q].y * s1);
+        Fz[2 * n + 1] = mkc<R>(xb[q].x * s1, xb[q].y * s1);
+        Fr[2 * n] = zr[q].x * s1;
+        Fr[2 * n + 1] = zr[q].y * s1;
+      }
     }
   }
 #pragma unroll
@@ -545,29 +583,39 @@ __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs&
     xa[q] = mkc<R>(xa[q].x * ze, xa[q].y * ze);
     xb[q] = mkc<R>(xb[q].x * zo, xb[q].y * zo);
   }
-  wave_fft2<M, Q, -1, R>(xa, xb, lds1, lds2, lane, twr);
+  fft2(xa, xb, Fwd{});
 #pragma unroll
   for (int q = 0; q < Q; ++q) {  // W[k] in xa, W[k + M] in xb
-    const cplx<R> t = cmul<R>(roots[lane + 64 * q], xb[q]);
+    const cplx<R> t = cmul<R>(roots[lane + T * q], xb[q]);
     const cplx<R> e = xa[q];
     xa[q] = mkc<R>(e.x + t.x, e.y + t.y);
     xb[q] = mkc<R>(e.x - t.x, e.y - t.y);
   }
 
   // split W into the half spectra of its real and imaginary parts.  For
-  // k = l + 64 q >= 1, NX - k = M + (M - k) and M - k = (64 - l) + 64 (Q-1-q)
-  // (lane l >= 1) or 64 (Q - q) (lane 0).
+  // k = l + T q >= 1, NX - k = M + (M - k): W[NX - k] is element M - k of
+  // xb.  One wavefront: M - k = (64 - l) + 64 (Q-1-q) (lane l >= 1) or
+  // 64 (Q - q) (lane 0), fetched from that lane; larger teams: through LDS.
   cplx<R>* O0 = reinterpret_cast<cplx<R>*>(a.O0);
   cplx<R>* O1 = reinterpret_cast<cplx<R>*>(a.O1);
+  if constexpr (!WAVE) {  // the transform ended with its LDS reads behind a barrier
+#pragma unroll
+    for (int q = 0; q < Q; ++q) lds1[lpad(lane + T * q)] = xb[q];
+    __syncthreads();
+  }
   const int partner = (64 - lane) & 63;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    if (64 * q >= g.KXp) break;  // uniform over the wavefront: the shuffles below run in every lane
+    if (T * q >= g.KXp) break;  // uniform over the workgroup: the shuffles below run in every lane
+    const int k = lane + T * q;
     cplx<R> mir;
-    mir.x = __shfl(xb[Q - 1 - q].x, partner, 64);
-    mir.y = __shfl(xb[Q - 1 - q].y, partner, 64);
-    if (lane == 0) mir = q == 0 ? xa[0] : xb[q == 0 ? 0 : Q - q];
-    const int k = lane + 64 * q;
+    if constexpr (WAVE) {
+      mir.x = __shfl(xb[Q - 1 - q].x, partner, 64);
+      mir.y = __shfl(xb[Q - 1 - q].y, partner, 64);
+      if (lane == 0) mir = q == 0 ? xa[0] : xb[q == 0 ? 0 : Q - q];
+    } else {
+      mir = k == 0 ? xa[0] : lds1[lpad(M - k)];
+    }
     if (k < g.KXp) {
       cplx<R> Pk = mkc<R>(0, 0), Qk = mkc<R>(0, 0);
       if (k < KX) {

@@ -152,7 +152,28 @@ __global__ void __launch_bounds__(256) k_vrowh(VortGeom g, VortRowArgs a) {
   const int grp = blockIdx.x;  // grid = ny / 4
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   cplx<float>* base = reinterpret_cast<cplx<float>*>(vsm) + 2 * w * SL;
-  vrowh_body<NX, Q>(g, a, grp * 4 + w, lane, base, base + SL);
+  vrowh_body<float, NX, Q>(g, a, grp * 4 + w, lane, base, base + SL);
+}
+
+// FP64 rows with half-length transforms: four rows per workgroup, a team of
+// two wavefronts each (M = NX/2 points at 8 per thread), the M-point twiddle
+// table in LDS after the rows' buffers.
+#ifndef FF_ROWH_F64
+#define FF_ROWH_F64 1  // FP64 2048-point row passes use the half-length kernel
+#endif
+template <int NX>
+constexpr bool rowh64_on() {
+  return FF_ROWH_F64 && NX == 2048;
+}
+template <int NX>
+__global__ void __launch_bounds__(4 * (NX / 16), 2) k_vrowh64(VortGeom g, VortRowArgs a) {
+  extern __shared__ unsigned char vsm[];
+  constexpr int M = NX / 2, Q = 8, T = M / Q;
+  constexpr int SL = TeamSched<M, Q>::LDS;
+  cplx<double>* base = reinterpret_cast<cplx<double>*>(vsm);
+  const cplx<double>* ltw = team_tw_load<M, Q, double, 2>(base + 8 * SL, a.tw);  // every other NX root
+  const int r = threadIdx.x / T, tid = threadIdx.x % T;
+  vrowh_body<double, NX, Q>(g, a, blockIdx.x * 4 + r, tid, base + 2 * r * SL, base + (2 * r + 1) * SL, ltw);
 }
 
 // ----------------------------------------------------------------- host side
@@ -201,8 +222,8 @@ static bool use3(const char* var, bool f32) {
   const int mode = e ? atoi(e) : 0;
   return f32 ? mode >= 2 : mode >= 1;
 }
-// FFHIP_ROWH=0 selects the two-row team kernel for FP32 rows where the
-// wavefront kernel applies (default on)
+// FFHIP_ROWH=0 selects the full-length team kernels for the rows where a
+// half-length kernel applies (FP32 1024/2048, FP64 2048; default on)
 static bool rowh_enabled() {
   static const bool on = [] {
     const char* e = getenv("FFHIP_ROWH");
@@ -272,10 +293,30 @@ static int launch_rowh(const VortGeom& g, const VortRowArgs& a, hipStream_t st) 
   return FFHIP_OK;
 }
 
+template <int N>
+static int launch_rowh64(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
+  constexpr int M = N / 2, Q = 8;
+  const size_t sh = ((size_t)8 * TeamSched<M, Q>::LDS + tw_slots<M, Q>()) * sizeof(cplx<double>);
+  auto kern = k_vrowh64<N>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    FF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)sh));
+    attr_set = true;
+  }
+  FF_CHECK(g.ny % 4 == 0, FFHIP_ERR_UNSUPPORTED, "FP64 half-length row kernel: ny = %d", g.ny);
+  prof_launch("vort_row", row_bytes<double>(g), kern, dim3(g.ny / 4), dim3(4 * (M / Q)), sh, st, g, a);
+  FF_LAUNCHED("k_vrowh64");
+  return FFHIP_OK;
+}
+
 template <typename R, int N>
 static int launch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
   if constexpr (sizeof(R) == 4 && rowh_on<N>()) {
     if (rowh_enabled()) return launch_rowh<N>(g, a, st);
+  }
+  if constexpr (sizeof(R) == 8 && rowh64_on<N>()) {
+    if (rowh_enabled()) return launch_rowh64<N>(g, a, st);
   }
   constexpr int P = vrow_pts<R, N>();
   const size_t sh = row_lds<R, N>();

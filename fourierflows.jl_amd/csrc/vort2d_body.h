@@ -420,7 +420,7 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
 // lds2 are the row's two buffers of TeamSched<M, Q>::LDS slots.
 // Team size T = M/Q: 64 (FP32) is one wavefront, with every twiddle in
 // registers (filled from the NX roots of unity in tw) and wavefront-level
-// exchanges; a larger team (FP64: 128 threads, two wavefronts) takes its
+// exchanges; a larger team (128 threads, two wavefronts) takes its
 // factors from the LDS table ltw, exchanges through workgroup barriers (all
 // rows of the workgroup in step) and fetches the split's mirrored
 // coefficient from LDS instead of another lane.  In the text above read
@@ -432,7 +432,6 @@ __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs&
   constexpr int T = M / Q;
   constexpr bool WAVE = T == 64;
   static_assert(T % 64 == 0, "whole wavefronts per row");
-  static_assert(WAVE || sizeof(R) == 8, "FP32 rows run one wavefront each");
   // zeta^, u^, v^ loaded up front in 16 B chunks (FP32 wavefront rows)
   constexpr bool PRE = WAVE && FF_ROWH_UVPRE;
   constexpr int CB = kVCB;

@@ -155,25 +155,26 @@ __global__ void __launch_bounds__(256) k_vrowh(VortGeom g, VortRowArgs a) {
   vrowh_body<float, NX, Q>(g, a, grp * 4 + w, lane, base, base + SL);
 }
 
-// FP64 rows with half-length transforms: four rows per workgroup, a team of
-// two wavefronts each (M = NX/2 points at 8 per thread), the M-point twiddle
-// table in LDS after the rows' buffers.
-#ifndef FF_ROWH_F64
-#define FF_ROWH_F64 1  // FP64 2048-point row passes use the half-length kernel
+// Rows with half-length transforms by teams of two wavefronts (128 threads:
+// FP64 2048-point rows at 8 points per thread, FP32 4096-point rows at 16),
+// four rows per workgroup, the M-point twiddle table in LDS after the rows'
+// buffers.
+#ifndef FF_ROWH_TEAM
+#define FF_ROWH_TEAM 1  // FP64 2048 and FP32 4096 row passes use the two-wavefront half-length kernel
 #endif
-template <int NX>
-constexpr bool rowh64_on() {
-  return FF_ROWH_F64 && NX == 2048;
+template <typename R, int NX>
+constexpr bool rowht_on() {
+  return FF_ROWH_TEAM && (sizeof(R) == 8 ? NX == 2048 : NX == 4096);
 }
-template <int NX>
-__global__ void __launch_bounds__(4 * (NX / 16), 2) k_vrowh64(VortGeom g, VortRowArgs a) {
+template <typename R, int NX>
+__global__ void __launch_bounds__(512, 2) k_vrowht(VortGeom g, VortRowArgs a) {
   extern __shared__ unsigned char vsm[];
-  constexpr int M = NX / 2, Q = 8, T = M / Q;
+  constexpr int M = NX / 2, T = 128, Q = M / T;
   constexpr int SL = TeamSched<M, Q>::LDS;
-  cplx<double>* base = reinterpret_cast<cplx<double>*>(vsm);
-  const cplx<double>* ltw = team_tw_load<M, Q, double, 2>(base + 8 * SL, a.tw);  // every other NX root
+  cplx<R>* base = reinterpret_cast<cplx<R>*>(vsm);
+  const cplx<R>* ltw = team_tw_load<M, Q, R, 2>(base + 8 * SL, a.tw);  // every other NX root
   const int r = threadIdx.x / T, tid = threadIdx.x % T;
-  vrowh_body<double, NX, Q>(g, a, blockIdx.x * 4 + r, tid, base + 2 * r * SL, base + (2 * r + 1) * SL, ltw);
+  vrowh_body<R, NX, Q>(g, a, blockIdx.x * 4 + r, tid, base + 2 * r * SL, base + (2 * r + 1) * SL, ltw);
 }
 
 // ----------------------------------------------------------------- host side
@@ -223,7 +224,7 @@ static bool use3(const char* var, bool f32) {
   return f32 ? mode >= 2 : mode >= 1;
 }
 // FFHIP_ROWH=0 selects the full-length team kernels for the rows where a
-// half-length kernel applies (FP32 1024/2048, FP64 2048; default on)
+// half-length kernel applies (FP32 1024/2048/4096, FP64 2048; default on)
 static bool rowh_enabled() {
   static const bool on = [] {
     const char* e = getenv("FFHIP_ROWH");
@@ -293,20 +294,20 @@ static int launch_rowh(const VortGeom& g, const VortRowArgs& a, hipStream_t st) 
   return FFHIP_OK;
 }
 
-template <int N>
-static int launch_rowh64(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
-  constexpr int M = N / 2, Q = 8;
-  const size_t sh = ((size_t)8 * TeamSched<M, Q>::LDS + tw_slots<M, Q>()) * sizeof(cplx<double>);
-  auto kern = k_vrowh64<N>;
+template <typename R, int N>
+static int launch_rowht(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
+  constexpr int M = N / 2, T = 128, Q = M / T;
+  const size_t sh = ((size_t)8 * TeamSched<M, Q>::LDS + tw_slots<M, Q>()) * sizeof(cplx<R>);
+  auto kern = k_vrowht<R, N>;
   static bool attr_set = false;
   if (!attr_set) {
     FF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)sh));
     attr_set = true;
   }
-  FF_CHECK(g.ny % 4 == 0, FFHIP_ERR_UNSUPPORTED, "FP64 half-length row kernel: ny = %d", g.ny);
-  prof_launch("vort_row", row_bytes<double>(g), kern, dim3(g.ny / 4), dim3(4 * (M / Q)), sh, st, g, a);
-  FF_LAUNCHED("k_vrowh64");
+  FF_CHECK(g.ny % 4 == 0, FFHIP_ERR_UNSUPPORTED, "half-length team row kernel: ny = %d", g.ny);
+  prof_launch("vort_row", row_bytes<R>(g), kern, dim3(g.ny / 4), dim3(4 * T), sh, st, g, a);
+  FF_LAUNCHED("k_vrowht");
   return FFHIP_OK;
 }
 
@@ -315,8 +316,8 @@ static int launch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
   if constexpr (sizeof(R) == 4 && rowh_on<N>()) {
     if (rowh_enabled()) return launch_rowh<N>(g, a, st);
   }
-  if constexpr (sizeof(R) == 8 && rowh64_on<N>()) {
-    if (rowh_enabled()) return launch_rowh64<N>(g, a, st);
+  if constexpr (rowht_on<R, N>()) {
+    if (rowh_enabled()) return launch_rowht<R, N>(g, a, st);
   }
   constexpr int P = vrow_pts<R, N>();
   const size_t sh = row_lds<R, N>();

@@ -61,16 +61,19 @@ template <int DIR, typename R> struct Bfly<16, DIR, R> {
 constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
 constexpr int cfull(int N, int P) { return (N % P == 0 && N >= P) ? 1 + cfull(N / P, P) : 0; }
 
+#ifndef FF_LDS_PAD_SHIFT
+#define FF_LDS_PAD_SHIFT 4  // one pad slot per 2^shift slots of a team's LDS buffer
+#endif
 template <int N, int P> struct TeamSched {
   static_assert((N & (N - 1)) == 0 && (P & (P - 1)) == 0, "powers of two only");
   static constexpr int T = N / P;
   static constexpr int NFULL = cfull(N, P);
   static constexpr int REM = N / cpow(P, NFULL);
   static constexpr int NPASS = NFULL + (REM > 1 ? 1 : 0);
-  static constexpr int LDS = N + N / 16;  // complex slots per team
+  static constexpr int LDS = N + (N >> FF_LDS_PAD_SHIFT);  // complex slots per team
 };
 
-__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+__device__ __forceinline__ int lpad(int i) { return i + (i >> FF_LDS_PAD_SHIFT); }
 
 #ifndef FF_LDS_BANKS
 #define FF_LDS_BANKS 32  // 4-byte LDS banks the slice rotation targets (0: no rotation)

@@ -99,8 +99,8 @@ static int build_items(ffhip_ensemble* e, EnsVariant v) {
     for (int c = 0; c < g.TX * kVCB; c += cpb) cols.push_back(make_int2(m, c));
     for (int y = 0; y < g.ny; y += rpb) rows.push_back(make_int2(m, y));
     cnt[m] = make_int2((g.ny + rpb - 1) / rpb, (g.TX * kVCB + cpb - 1) / cpb);
-    lc = std::max(lc, ((size_t)cpb * (size_t)team_slice(g.ny + g.ny / 16, (int)cs, cpb) + tw_slots_rt(g.ny, Py)) * cs);
-    lr = std::max(lr, ((size_t)rpb * (size_t)team_slice(g.nx + g.nx / 16, (int)cs, rpb) + tw_slots_rt(g.nx, Px)) * cs);
+    lc = std::max(lc, ((size_t)cpb * (size_t)team_slice(g.ny + (g.ny >> FF_LDS_PAD_SHIFT), (int)cs, cpb) + tw_slots_rt(g.ny, Py)) * cs);
+    lr = std::max(lr, ((size_t)rpb * (size_t)team_slice(g.nx + (g.nx >> FF_LDS_PAD_SHIFT), (int)cs, rpb) + tw_slots_rt(g.nx, Px)) * cs);
   }
   FF_CHECK(lc <= (size_t)max_lds_per_block() && lr <= (size_t)max_lds_per_block(), FFHIP_ERR_UNSUPPORTED,
            "ensemble variant needs more LDS than a workgroup has");

@@ -107,6 +107,33 @@ def test_fp64_gradient_512_matches_oracle(ff):
     assert err < 1e-11, err
 
 
+@pytest.mark.parametrize("T,nx,tol", [(np.float64, 2048, 1e-11), (np.float32, 2048, 1e-4), (np.float32, 1024, 1e-4)])
+def test_gradient_halflength_rows_matches_oracle(ff, T, nx, tol):
+    """Rows of 1024 (FP32) and 2048 points run the half-length row kernels
+    (one wavefront per FP32 row, two per FP64 row), whose stored stage fields
+    (u + iv and zeta, normalised) feed the adjoint sweep: the gradient of an
+    nx x 64 grid against the FP64 oracle adjoint."""
+    import torch
+
+    ny, nu, dt, nsteps, ncheck = 64, 1e-2, 1e-3, 4, 2
+    prob = ff.TwoDVorticityProblem(nx=nx, ny=ny, nu=nu, dt=dt, T=T)
+    assert prob.eqn.calcN.fused
+    g = prob.grid
+    z0 = torch.zeros(g.ny, g.nx, dtype=prob.sol.real.dtype, device="cuda")
+    ff.ldiv(z0, g.rfftplan, ff.vorticity_ic(g, seed=1))
+    prob.sol.copy_(ff.vorticity_ic(g, seed=2))
+    ff.stepforward(prob, nsteps)
+    zobs = torch.zeros_like(z0)
+    ff.ldiv(zobs, g.rfftplan, prob.sol)
+    J, grad = ff.adjoint.misfit_gradient(prob, z0, zobs, nsteps, ncheck)
+    og_ = og.TwoDGrid(nx, 2 * np.pi, ny, 2 * np.pi, x0=0, y0=0)
+    adj = oa.ETDRK4Adjoint(og_, oph.vorticity_linear(og_, nu=nu), dt)
+    Jr, gr = adj.gradient(_host(z0).astype(np.float64), _host(zobs).astype(np.float64), nsteps, ncheck)
+    assert abs(J - Jr) < max(tol, 1e-12) * Jr * 10, (J, Jr)
+    err = np.linalg.norm(_host(grad).astype(np.float64) - gr) / np.linalg.norm(gr)
+    assert err < tol, err
+
+
 def test_adjoint_repeatable_and_checkpoint_independent(ff):
     prob, z0, zobs = _setup(ff, 128, 1e-2, 0.01, 24, np.float64)
     J1, g1 = ff.adjoint.misfit_gradient(prob, z0, zobs, 24, 6)

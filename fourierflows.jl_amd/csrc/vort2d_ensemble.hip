@@ -38,6 +38,10 @@ constexpr int kEnsChunk = 10;  // steps per captured graph
 
 using namespace ffhip;
 
+#ifndef FF_ENS_STREAMS
+#define FF_ENS_STREAMS 2  // the step graph runs the members as two groups on two streams (1: one stream)
+#endif
+
 struct EnsVariant {
   int bd = 128;
   int tcol = kEnsDefaultThr, trow = kEnsDefaultThr;  // sizes from these up take 16 points per thread
@@ -57,10 +61,13 @@ struct ffhip_ensemble {
   int autotune = 0, tuned = 0;
   EnsVariant var;
   double tuned_ms = 0;
-  void* items = nullptr;  // int2[ncol + nrow] for var
+  void* items = nullptr;  // int2[ncol + nrow] for var: columns of group A, of group B, rows of A, of B
   int ncol = 0, nrow = 0;
+  int ncolA = 0, nrowA = 0;  // items of group A (the graph runs groups A and B on two streams)
   size_t lds_col = 0, lds_row = 0;
   hipStream_t cap = nullptr;
+  hipStream_t cap2 = nullptr;                    // group B's capture stream
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraphExec_t exec = nullptr;
   int items_gen = 0, graph_gen = -1;  // the graph holds the item-table pointer it was captured with
   void* sync = nullptr;    // stage-pair counters: rows_done[count], cols_done[count], err
@@ -86,18 +93,23 @@ static int build_items(ffhip_ensemble* e, EnsVariant v) {
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return (int64_t)e->geo[a].nx * e->geo[a].ny > (int64_t)e->geo[b].nx * e->geo[b].ny;
   });
-  std::vector<int2> cols, rows;
+  std::vector<int2> cols, rows, colsB, rowsB;
   std::vector<int2> cnt(e->count, make_int2(0, 0));
   size_t lc = 0, lr = 0;
   const size_t cs = cplx_size(e->prec);
-  for (int m : order) {
+  for (int im = 0; im < (int)order.size(); ++im) {
+    const int m = order[im];
+    // members alternate between groups A and B in size order (balanced work)
+    const bool b = FF_ENS_STREAMS == 2 && (im & 1);
+    std::vector<int2>& mc = b ? colsB : cols;
+    std::vector<int2>& mr = b ? rowsB : rows;
     const VortGeom& g = e->geo[m];
     const int Py = ens_pts(g.ny, v.tcol), Px = ens_pts(g.nx, v.trow);
     const int cpb = v.bd / (g.ny / Py), rpb = v.bd / (g.nx / Px);
     FF_CHECK(cpb >= 1 && rpb >= 1, FFHIP_ERR_UNSUPPORTED, "variant %d/%d/%d cannot hold a %d x %d member", v.bd,
              v.tcol, v.trow, g.nx, g.ny);
-    for (int c = 0; c < g.TX * kVCB; c += cpb) cols.push_back(make_int2(m, c));
-    for (int y = 0; y < g.ny; y += rpb) rows.push_back(make_int2(m, y));
+    for (int c = 0; c < g.TX * kVCB; c += cpb) mc.push_back(make_int2(m, c));
+    for (int y = 0; y < g.ny; y += rpb) mr.push_back(make_int2(m, y));
     cnt[m] = make_int2((g.ny + rpb - 1) / rpb, (g.TX * kVCB + cpb - 1) / cpb);
     lc = std::max(lc, ((size_t)cpb * (size_t)team_slice(g.ny + (g.ny >> FF_LDS_PAD_SHIFT), (int)cs, cpb) + tw_slots_rt(g.ny, Py)) * cs);
     lr = std::max(lr, ((size_t)rpb * (size_t)team_slice(g.nx + (g.nx >> FF_LDS_PAD_SHIFT), (int)cs, rpb) + tw_slots_rt(g.nx, Px)) * cs);
@@ -106,6 +118,10 @@ static int build_items(ffhip_ensemble* e, EnsVariant v) {
            "ensemble variant needs more LDS than a workgroup has");
   dev_free(e->items);
   e->items = nullptr;
+  e->ncolA = (int)cols.size();
+  e->nrowA = (int)rows.size();
+  cols.insert(cols.end(), colsB.begin(), colsB.end());
+  rows.insert(rows.end(), rowsB.begin(), rowsB.end());
   std::vector<int2> all(cols);
   all.insert(all.end(), rows.begin(), rows.end());
   FF_TRY(dev_alloc(&e->items, all.size() * sizeof(int2)));
@@ -136,16 +152,20 @@ static int set_lds(K kern, size_t bytes) {
 
 // which: 0..4 column stage, 5 row pass.  attr_only: set the kernel's LDS
 // limit without launching (done outside any capture).
+// part: 0 group A, 1 group B, 2 every member
 template <typename R, int BD, int TC, int TR>
-static int ens_pass(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only) {
+static int ens_pass(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only, int part) {
   const EnsMember* mem = reinterpret_cast<const EnsMember*>(e->dmem);
   const int2* items = reinterpret_cast<const int2*>(e->items);
   static const char* names[6] = {"ens_col_stage0", "ens_col_stage1", "ens_col_stage2",
                                  "ens_col_stage3", "ens_col_stage4", "ens_row"};
+  const int c0 = part == 1 ? e->ncolA : 0, nc = part == 0 ? e->ncolA : (part == 1 ? e->ncol - e->ncolA : e->ncol);
+  const int r0 = part == 1 ? e->nrowA : 0, nr = part == 0 ? e->nrowA : (part == 1 ? e->nrow - e->nrowA : e->nrow);
   if (which == 5) {
     auto k = k_ens_row<R, BD, TR>;
     if (attr_only) return set_lds(k, e->lds_row);
-    prof_launch(names[5], e->bytes[5], k, dim3(e->nrow), dim3(BD), e->lds_row, st, mem, items + e->ncol);
+    if (nr == 0) return FFHIP_OK;
+    prof_launch(names[5], e->bytes[5], k, dim3(nr), dim3(BD), e->lds_row, st, mem, items + e->ncol + r0);
     FF_LAUNCHED("k_ens_row");
     return FFHIP_OK;
   }
@@ -158,18 +178,20 @@ static int ens_pass(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only
     default: k = k_ens_col<R, BD, TC, 4>; break;
   }
   if (attr_only) return set_lds(k, e->lds_col);
-  prof_launch(names[which], e->bytes[which], k, dim3(e->ncol), dim3(BD), e->lds_col, st, mem, items);
+  if (nc == 0) return FFHIP_OK;
+  prof_launch(names[which], e->bytes[which], k, dim3(nc), dim3(BD), e->lds_col, st, mem, items + c0);
   FF_LAUNCHED("k_ens_col");
   return FFHIP_OK;
 }
 
 template <typename R>
-static int ens_pass_v(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only) {
+static int ens_pass_v(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only, int part) {
   // column passes use tcol, the row pass trow: instantiate each pass with its own threshold only
   const EnsVariant& v = e->var;
   const int t = which == 5 ? v.trow : v.tcol;
-#define FF_ENS_T(BD, T)                                                                                      \
-  (which == 5 ? ens_pass<R, BD, 2048, T>(e, which, st, attr_only) : ens_pass<R, BD, T, 2048>(e, which, st, attr_only))
+#define FF_ENS_T(BD, T)                                                           \
+  (which == 5 ? ens_pass<R, BD, 2048, T>(e, which, st, attr_only, part)         \
+              : ens_pass<R, BD, T, 2048>(e, which, st, attr_only, part))
 #define FF_ENS_BD(BD)                                                           \
   switch (t) {                                                                  \
     case 128: return FF_ENS_T(BD, 128);                                         \
@@ -185,9 +207,9 @@ static int ens_pass_v(ffhip_ensemble* e, int which, hipStream_t st, bool attr_on
 #undef FF_ENS_T
 }
 
-static int ens_launch(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only = false) {
-  return e->prec == FFHIP_F32 ? ens_pass_v<float>(e, which, st, attr_only)
-                              : ens_pass_v<double>(e, which, st, attr_only);
+static int ens_launch(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only = false, int part = 2) {
+  return e->prec == FFHIP_F32 ? ens_pass_v<float>(e, which, st, attr_only, part)
+                              : ens_pass_v<double>(e, which, st, attr_only, part);
 }
 
 static EnsPairArgs pair_args(const ffhip_ensemble* e, int stage) {
@@ -218,6 +240,17 @@ static int ens_step(ffhip_ensemble* e, hipStream_t st) {
   return FFHIP_OK;
 }
 
+// one step of group A on stA and of group B on stB (per-pass launches)
+static int ens_step_split(ffhip_ensemble* e, hipStream_t stA, hipStream_t stB) {
+  for (int stage = 1; stage <= 4; ++stage) {
+    FF_TRY(ens_launch(e, 5, stA, false, 0));
+    FF_TRY(ens_launch(e, stage, stA, false, 0));
+    FF_TRY(ens_launch(e, 5, stB, false, 1));
+    FF_TRY(ens_launch(e, stage, stB, false, 1));
+  }
+  return FFHIP_OK;
+}
+
 // the stage-pair counters start at zero (a fill kernel, outside any capture)
 static __global__ void k_zero_ints(int* p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
@@ -236,9 +269,27 @@ static void drop_graph(ffhip_ensemble* e) {
 static int capture_graph(ffhip_ensemble* e) {
   drop_graph(e);
   if (!e->cap) FF_HIP(hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking));
+  // two member groups on two streams (forked from and joined back into the
+  // capture stream): one group's passes fill the other's launch gaps and tails
+  const bool split = !e->var.pair && e->ncolA < e->ncol;
+  if (split) {
+    if (!e->cap2) FF_HIP(hipStreamCreateWithFlags(&e->cap2, hipStreamNonBlocking));
+    if (!e->ev_fork) FF_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+    if (!e->ev_join) FF_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
+  }
   FF_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
   int rc = FFHIP_OK;
-  for (int k = 0; k < kEnsChunk && rc == FFHIP_OK; ++k) rc = ens_step(e, e->cap);
+  if (split) {
+    hipError_t ef = hipEventRecord(e->ev_fork, e->cap);
+    if (ef == hipSuccess) ef = hipStreamWaitEvent(e->cap2, e->ev_fork, 0);
+    if (ef != hipSuccess) rc = fail(FFHIP_ERR_HIP, "ensemble graph fork: %s", hipGetErrorString(ef));
+    for (int k = 0; k < kEnsChunk && rc == FFHIP_OK; ++k) rc = ens_step_split(e, e->cap, e->cap2);
+    hipError_t ej = hipEventRecord(e->ev_join, e->cap2);
+    if (ej == hipSuccess) ej = hipStreamWaitEvent(e->cap, e->ev_join, 0);
+    if (ej != hipSuccess && rc == FFHIP_OK) rc = fail(FFHIP_ERR_HIP, "ensemble graph join: %s", hipGetErrorString(ej));
+  } else {
+    for (int k = 0; k < kEnsChunk && rc == FFHIP_OK; ++k) rc = ens_step(e, e->cap);
+  }
   hipGraph_t graph = nullptr;
   hipError_t ee = hipStreamEndCapture(e->cap, &graph);
   if (rc != FFHIP_OK) {
@@ -357,6 +408,9 @@ static void ens_free(ffhip_ensemble* e) {
   if (!e) return;
   drop_graph(e);
   if (e->cap) (void)hipStreamDestroy(e->cap);
+  if (e->cap2) (void)hipStreamDestroy(e->cap2);
+  if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+  if (e->ev_join) (void)hipEventDestroy(e->ev_join);
   dev_free(e->pool);
   dev_free(e->dmem);
   dev_free(e->items);

@@ -39,7 +39,7 @@ constexpr int kEnsChunk = 10;  // steps per captured graph
 using namespace ffhip;
 
 #ifndef FF_ENS_STREAMS
-#define FF_ENS_STREAMS 2  // the step graph runs the members as two groups on two streams (1: one stream)
+#define FF_ENS_STREAMS 2  // the step graph runs the members as this many groups on as many streams
 #endif
 
 struct EnsVariant {
@@ -63,11 +63,12 @@ struct ffhip_ensemble {
   double tuned_ms = 0;
   void* items = nullptr;  // int2[ncol + nrow] for var: columns of group A, of group B, rows of A, of B
   int ncol = 0, nrow = 0;
-  int ncolA = 0, nrowA = 0;  // items of group A (the graph runs groups A and B on two streams)
+  int ngroup = 1;                          // member groups (the graph runs each on its own stream)
+  int col0[FF_ENS_STREAMS + 1] = {}, row0[FF_ENS_STREAMS + 1] = {};  // first column / row item of each group
   size_t lds_col = 0, lds_row = 0;
   hipStream_t cap = nullptr;
-  hipStream_t cap2 = nullptr;                    // group B's capture stream
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t capg[FF_ENS_STREAMS] = {};  // capture streams of groups 1.. (group 0 runs on cap)
+  hipEvent_t ev_fork = nullptr, ev_join[FF_ENS_STREAMS] = {};
   hipGraphExec_t exec = nullptr;
   int items_gen = 0, graph_gen = -1;  // the graph holds the item-table pointer it was captured with
   void* sync = nullptr;    // stage-pair counters: rows_done[count], cols_done[count], err
@@ -93,16 +94,15 @@ static int build_items(ffhip_ensemble* e, EnsVariant v) {
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return (int64_t)e->geo[a].nx * e->geo[a].ny > (int64_t)e->geo[b].nx * e->geo[b].ny;
   });
-  std::vector<int2> cols, rows, colsB, rowsB;
+  std::vector<std::vector<int2>> gcols(FF_ENS_STREAMS), grows(FF_ENS_STREAMS);
   std::vector<int2> cnt(e->count, make_int2(0, 0));
   size_t lc = 0, lr = 0;
   const size_t cs = cplx_size(e->prec);
   for (int im = 0; im < (int)order.size(); ++im) {
     const int m = order[im];
-    // members alternate between groups A and B in size order (balanced work)
-    const bool b = FF_ENS_STREAMS == 2 && (im & 1);
-    std::vector<int2>& mc = b ? colsB : cols;
-    std::vector<int2>& mr = b ? rowsB : rows;
+    // members are dealt to the groups in size order (balanced work)
+    std::vector<int2>& mc = gcols[im % FF_ENS_STREAMS];
+    std::vector<int2>& mr = grows[im % FF_ENS_STREAMS];
     const VortGeom& g = e->geo[m];
     const int Py = ens_pts(g.ny, v.tcol), Px = ens_pts(g.nx, v.trow);
     const int cpb = v.bd / (g.ny / Py), rpb = v.bd / (g.nx / Px);
@@ -118,10 +118,16 @@ static int build_items(ffhip_ensemble* e, EnsVariant v) {
            "ensemble variant needs more LDS than a workgroup has");
   dev_free(e->items);
   e->items = nullptr;
-  e->ncolA = (int)cols.size();
-  e->nrowA = (int)rows.size();
-  cols.insert(cols.end(), colsB.begin(), colsB.end());
-  rows.insert(rows.end(), rowsB.begin(), rowsB.end());
+  std::vector<int2> cols, rows;
+  e->ngroup = std::min(FF_ENS_STREAMS, e->count);
+  for (int gi = 0; gi < FF_ENS_STREAMS; ++gi) {
+    e->col0[gi] = (int)cols.size();
+    e->row0[gi] = (int)rows.size();
+    cols.insert(cols.end(), gcols[gi].begin(), gcols[gi].end());
+    rows.insert(rows.end(), grows[gi].begin(), grows[gi].end());
+  }
+  e->col0[FF_ENS_STREAMS] = (int)cols.size();
+  e->row0[FF_ENS_STREAMS] = (int)rows.size();
   std::vector<int2> all(cols);
   all.insert(all.end(), rows.begin(), rows.end());
   FF_TRY(dev_alloc(&e->items, all.size() * sizeof(int2)));
@@ -152,15 +158,15 @@ static int set_lds(K kern, size_t bytes) {
 
 // which: 0..4 column stage, 5 row pass.  attr_only: set the kernel's LDS
 // limit without launching (done outside any capture).
-// part: 0 group A, 1 group B, 2 every member
+// part: a group's index, or -1 for every member
 template <typename R, int BD, int TC, int TR>
 static int ens_pass(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only, int part) {
   const EnsMember* mem = reinterpret_cast<const EnsMember*>(e->dmem);
   const int2* items = reinterpret_cast<const int2*>(e->items);
   static const char* names[6] = {"ens_col_stage0", "ens_col_stage1", "ens_col_stage2",
                                  "ens_col_stage3", "ens_col_stage4", "ens_row"};
-  const int c0 = part == 1 ? e->ncolA : 0, nc = part == 0 ? e->ncolA : (part == 1 ? e->ncol - e->ncolA : e->ncol);
-  const int r0 = part == 1 ? e->nrowA : 0, nr = part == 0 ? e->nrowA : (part == 1 ? e->nrow - e->nrowA : e->nrow);
+  const int c0 = part < 0 ? 0 : e->col0[part], nc = part < 0 ? e->ncol : e->col0[part + 1] - c0;
+  const int r0 = part < 0 ? 0 : e->row0[part], nr = part < 0 ? e->nrow : e->row0[part + 1] - r0;
   if (which == 5) {
     auto k = k_ens_row<R, BD, TR>;
     if (attr_only) return set_lds(k, e->lds_row);
@@ -207,7 +213,7 @@ static int ens_pass_v(ffhip_ensemble* e, int which, hipStream_t st, bool attr_on
 #undef FF_ENS_T
 }
 
-static int ens_launch(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only = false, int part = 2) {
+static int ens_launch(ffhip_ensemble* e, int which, hipStream_t st, bool attr_only = false, int part = -1) {
   return e->prec == FFHIP_F32 ? ens_pass_v<float>(e, which, st, attr_only, part)
                               : ens_pass_v<double>(e, which, st, attr_only, part);
 }
@@ -240,14 +246,13 @@ static int ens_step(ffhip_ensemble* e, hipStream_t st) {
   return FFHIP_OK;
 }
 
-// one step of group A on stA and of group B on stB (per-pass launches)
-static int ens_step_split(ffhip_ensemble* e, hipStream_t stA, hipStream_t stB) {
-  for (int stage = 1; stage <= 4; ++stage) {
-    FF_TRY(ens_launch(e, 5, stA, false, 0));
-    FF_TRY(ens_launch(e, stage, stA, false, 0));
-    FF_TRY(ens_launch(e, 5, stB, false, 1));
-    FF_TRY(ens_launch(e, stage, stB, false, 1));
-  }
+// one step of every group, group g on sts[g] (per-pass launches)
+static int ens_step_split(ffhip_ensemble* e, const hipStream_t* sts) {
+  for (int stage = 1; stage <= 4; ++stage)
+    for (int gi = 0; gi < e->ngroup; ++gi) {
+      FF_TRY(ens_launch(e, 5, sts[gi], false, gi));
+      FF_TRY(ens_launch(e, stage, sts[gi], false, gi));
+    }
   return FFHIP_OK;
 }
 
@@ -269,24 +274,32 @@ static void drop_graph(ffhip_ensemble* e) {
 static int capture_graph(ffhip_ensemble* e) {
   drop_graph(e);
   if (!e->cap) FF_HIP(hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking));
-  // two member groups on two streams (forked from and joined back into the
-  // capture stream): one group's passes fill the other's launch gaps and tails
-  const bool split = !e->var.pair && e->ncolA < e->ncol;
+  // member groups on their own streams (forked from and joined back into
+  // the capture stream): one group's passes fill the others' launch gaps and
+  // partly filled tails
+  const bool split = !e->var.pair && e->ngroup > 1;
+  hipStream_t sts[FF_ENS_STREAMS] = {};
   if (split) {
-    if (!e->cap2) FF_HIP(hipStreamCreateWithFlags(&e->cap2, hipStreamNonBlocking));
+    sts[0] = e->cap;
+    for (int gi = 1; gi < e->ngroup; ++gi) {
+      if (!e->capg[gi]) FF_HIP(hipStreamCreateWithFlags(&e->capg[gi], hipStreamNonBlocking));
+      if (!e->ev_join[gi]) FF_HIP(hipEventCreateWithFlags(&e->ev_join[gi], hipEventDisableTiming));
+      sts[gi] = e->capg[gi];
+    }
     if (!e->ev_fork) FF_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
-    if (!e->ev_join) FF_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
   }
   FF_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
   int rc = FFHIP_OK;
   if (split) {
     hipError_t ef = hipEventRecord(e->ev_fork, e->cap);
-    if (ef == hipSuccess) ef = hipStreamWaitEvent(e->cap2, e->ev_fork, 0);
+    for (int gi = 1; gi < e->ngroup && ef == hipSuccess; ++gi) ef = hipStreamWaitEvent(sts[gi], e->ev_fork, 0);
     if (ef != hipSuccess) rc = fail(FFHIP_ERR_HIP, "ensemble graph fork: %s", hipGetErrorString(ef));
-    for (int k = 0; k < kEnsChunk && rc == FFHIP_OK; ++k) rc = ens_step_split(e, e->cap, e->cap2);
-    hipError_t ej = hipEventRecord(e->ev_join, e->cap2);
-    if (ej == hipSuccess) ej = hipStreamWaitEvent(e->cap, e->ev_join, 0);
-    if (ej != hipSuccess && rc == FFHIP_OK) rc = fail(FFHIP_ERR_HIP, "ensemble graph join: %s", hipGetErrorString(ej));
+    for (int k = 0; k < kEnsChunk && rc == FFHIP_OK; ++k) rc = ens_step_split(e, sts);
+    for (int gi = 1; gi < e->ngroup; ++gi) {  // join every group back (also after an error: the capture must end)
+      hipError_t ej = hipEventRecord(e->ev_join[gi], sts[gi]);
+      if (ej == hipSuccess) ej = hipStreamWaitEvent(e->cap, e->ev_join[gi], 0);
+      if (ej != hipSuccess && rc == FFHIP_OK) rc = fail(FFHIP_ERR_HIP, "ensemble graph join: %s", hipGetErrorString(ej));
+    }
   } else {
     for (int k = 0; k < kEnsChunk && rc == FFHIP_OK; ++k) rc = ens_step(e, e->cap);
   }
@@ -408,9 +421,11 @@ static void ens_free(ffhip_ensemble* e) {
   if (!e) return;
   drop_graph(e);
   if (e->cap) (void)hipStreamDestroy(e->cap);
-  if (e->cap2) (void)hipStreamDestroy(e->cap2);
+  for (int gi = 0; gi < FF_ENS_STREAMS; ++gi) {
+    if (e->capg[gi]) (void)hipStreamDestroy(e->capg[gi]);
+    if (e->ev_join[gi]) (void)hipEventDestroy(e->ev_join[gi]);
+  }
   if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
-  if (e->ev_join) (void)hipEventDestroy(e->ev_join);
   dev_free(e->pool);
   dev_free(e->dmem);
   dev_free(e->items);

@@ -212,20 +212,25 @@ def tracer_timing(ff, torch, n=256, nsteps=300, dt=0.0025, kappa=1e-3):
 
 
 def vort4096_timing(ff, torch, steps=50):
-    """The same hyperviscous vorticity problem at 4096² in FP32 (the size the
-    reference's docs time: docs/src/assets/CPUversusGPU.jl:11, 23.724 s per 100
-    GPU steps): fused steps/s over `steps` steps."""
+    """The same hyperviscous vorticity problem at 4096² in FP32 and in FP64
+    (the size the reference's docs time, in its default FP64:
+    docs/src/assets/CPUversusGPU.jl:11, 23.724 s per 100 GPU steps): fused
+    steps/s over `steps` steps."""
     import numpy as np
 
     n = 4096
-    p = ff.TwoDVorticityProblem(nx=n, nu_n=nu8_for(n, DT), n_hyper=4, mu=MU, dt=DT, T=np.float32)
-    if not p.eqn.calcN.fused:
-        return None
-    p.sol.copy_(ff.vorticity_ic(p.grid, seed=1))
-    t = time_steps(ff, torch, p, steps, 5)
-    return {"workload": f"2D vorticity {n}^2 FP32, nu8|k|^8 + mu=0.01, dt=1e-3", "steps": steps,
-            "steps_per_s": round(steps / t, 2), "ms_per_step": round(1e3 * t / steps, 4),
-            "reference_published_gpu_steps_per_s": round(100 / 23.724, 3)}
+    out = {"workload": f"2D vorticity {n}^2, nu8|k|^8 + mu=0.01, dt=1e-3", "steps": steps,
+           "reference_published_gpu_steps_per_s": round(100 / 23.724, 3)}
+    for T, key in ((np.float32, "fp32"), (np.float64, "fp64")):
+        p = ff.TwoDVorticityProblem(nx=n, nu_n=nu8_for(n, DT), n_hyper=4, mu=MU, dt=DT, T=T)
+        if not p.eqn.calcN.fused:
+            return None
+        p.sol.copy_(ff.vorticity_ic(p.grid, seed=1).to(p.sol.dtype))
+        t = time_steps(ff, torch, p, steps, 5)
+        out[key] = {"steps_per_s": round(steps / t, 2), "ms_per_step": round(1e3 * t / steps, 4)}
+        del p
+    out["steps_per_s"] = out["fp32"]["steps_per_s"]
+    return out
 
 
 def fft_plan_timing(ff, torch, n=N, reps=50):

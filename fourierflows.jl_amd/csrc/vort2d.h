@@ -98,7 +98,7 @@ constexpr int vpts_row(int n) { return n >= 256 ? 16 : 8; }
 #endif
 template <typename R, int N>
 constexpr int vcol_pts() {
-  if constexpr (sizeof(R) == 8) return FF_COL_P8_F64 && N == 2048 ? 8 : vpts_col(N);
+  if constexpr (sizeof(R) == 8) return N == 4096 || (FF_COL_P8_F64 && N == 2048) ? 8 : vpts_col(N);
   return N >= 256 && (FF_COL_P8_ALWAYS || TeamSched<N, 8>::NPASS == TeamSched<N, 16>::NPASS) ? 8 : vpts_col(N);
 }
 // The same rule for the single-team row passes, forward and adjoint (512^2

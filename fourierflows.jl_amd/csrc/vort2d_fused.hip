@@ -62,22 +62,31 @@ constexpr bool row_pair() {
 #define FF_COL_WAVES_F64 1
 #endif
 
-// STAGE 0: y-inverse of U only.  STAGE 1..4: ETDRK4 stages.  kCBK columns per
-// workgroup; the kCSUB workgroups of one 4-column tile run on the same XCD.
+// Columns per workgroup: kCBK, but one for FP64 4096-point columns (a
+// column's buffer and twiddle table take 135 KB of LDS).
+template <typename R, int NY>
+constexpr int col_cbk() {
+  return sizeof(R) == 8 && NY == 4096 ? 1 : kCBK;
+}
+
+// STAGE 0: y-inverse of U only.  STAGE 1..4: ETDRK4 stages.  col_cbk columns
+// per workgroup; the workgroups of one 4-column tile run on the same XCD.
 template <typename R, int NY, int P, int STAGE>
-__global__ void __launch_bounds__(kCBK*(NY / P), sizeof(R) == 4 ? FF_COL_WAVES : FF_COL_WAVES_F64) k_vcol(VortGeom g, VortColArgs a) {
+__global__ void __launch_bounds__((col_cbk<R, NY>() * (NY / P)), (sizeof(R) == 4 ? FF_COL_WAVES : FF_COL_WAVES_F64))
+    k_vcol(VortGeom g, VortColArgs a) {
   extern __shared__ unsigned char vsm[];
+  constexpr int CB = col_cbk<R, NY>();
   int tx, sub;
-  xcd_split(blockIdx.x, kCSUB, &tx, &sub);
+  xcd_split(blockIdx.x, kVCB / CB, &tx, &sub);
   if (tx >= g.TX) return;  // grid is rounded up to whole XCD groups
   const int t = threadIdx.x;
-  const int cc = t % kCBK;
-  constexpr int SL = team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), kCBK);
+  const int cc = t % CB;
+  constexpr int SL = team_slice(TeamSched<NY, P>::LDS, sizeof(cplx<R>), CB);
   constexpr bool pair = col_pair<R, NY>();  // second buffers after the first, then the table
   cplx<R>* base = reinterpret_cast<cplx<R>*>(vsm);
-  const cplx<R>* tw = team_tw_load<NY, P, R>(base + (pair ? 2 : 1) * kCBK * SL, a.tw);
-  vcol_body<R, NY, P, STAGE, pair>(g, a, tx, sub * kCBK + cc, t / kCBK, base + cc * SL, tw, true,
-                                   pair ? base + (kCBK + cc) * SL : nullptr);
+  const cplx<R>* tw = team_tw_load<NY, P, R>(base + (pair ? 2 : 1) * CB * SL, a.tw);
+  vcol_body<R, NY, P, STAGE, pair>(g, a, tx, sub * CB + cc, t / CB, base + cc * SL, tw, true,
+                                   pair ? base + (CB + cc) * SL : nullptr);
 }
 
 // One column per workgroup of three teams (vcol3_body); the four columns of a
@@ -164,17 +173,28 @@ __global__ void __launch_bounds__(256) k_vrowh(VortGeom g, VortRowArgs a) {
 #endif
 template <typename R, int NX>
 constexpr bool rowht_on() {
-  return FF_ROWH_TEAM && (sizeof(R) == 8 ? NX == 2048 : NX == 4096);
+  return FF_ROWH_TEAM && (NX == 4096 || (sizeof(R) == 8 && NX == 2048));
+}
+// team size and rows per workgroup: FP64 4096-point rows take teams of
+// four wavefronts (2048-point transforms at 8 points per thread), one row
+// per workgroup (102 KB of LDS); the others two-wavefront teams, four rows
+template <typename R, int NX>
+constexpr int rowht_team() {
+  return sizeof(R) == 8 && NX == 4096 ? 256 : 128;
 }
 template <typename R, int NX>
-__global__ void __launch_bounds__(512, 2) k_vrowht(VortGeom g, VortRowArgs a) {
+constexpr int rowht_rows() {
+  return sizeof(R) == 8 && NX == 4096 ? 1 : 4;
+}
+template <typename R, int NX>
+__global__ void __launch_bounds__((rowht_team<R, NX>() * rowht_rows<R, NX>()), 2) k_vrowht(VortGeom g, VortRowArgs a) {
   extern __shared__ unsigned char vsm[];
-  constexpr int M = NX / 2, T = 128, Q = M / T;
+  constexpr int M = NX / 2, T = rowht_team<R, NX>(), Q = M / T, RPW = rowht_rows<R, NX>();
   constexpr int SL = TeamSched<M, Q>::LDS;
   cplx<R>* base = reinterpret_cast<cplx<R>*>(vsm);
-  const cplx<R>* ltw = team_tw_load<M, Q, R, 2>(base + 8 * SL, a.tw);  // every other NX root
+  const cplx<R>* ltw = team_tw_load<M, Q, R, 2>(base + 2 * RPW * SL, a.tw);  // every other NX root
   const int r = threadIdx.x / T, tid = threadIdx.x % T;
-  vrowh_body<R, NX, Q>(g, a, blockIdx.x * 4 + r, tid, base + 2 * r * SL, base + (2 * r + 1) * SL, ltw);
+  vrowh_body<R, NX, Q>(g, a, blockIdx.x * RPW + r, tid, base + 2 * r * SL, base + (2 * r + 1) * SL, ltw);
 }
 
 // ----------------------------------------------------------------- host side
@@ -182,8 +202,9 @@ static bool pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 template <typename R, int N>
 static size_t col_lds() {
-  constexpr size_t sl = team_slice(TeamSched<N, vcol_pts<R, N>()>::LDS, sizeof(cplx<R>), kCBK);
-  return ((col_pair<R, N>() ? 2 : 1) * kCBK * sl + tw_slots<N, vcol_pts<R, N>()>()) * sizeof(cplx<R>);
+  constexpr int CB = col_cbk<R, N>();
+  constexpr size_t sl = team_slice(TeamSched<N, vcol_pts<R, N>()>::LDS, sizeof(cplx<R>), CB);
+  return ((col_pair<R, N>() ? 2 : 1) * CB * sl + tw_slots<N, vcol_pts<R, N>()>()) * sizeof(cplx<R>);
 }
 
 template <typename R, int N>
@@ -207,8 +228,9 @@ static int launch_col(const VortGeom& g, const VortColArgs& a, hipStream_t st) {
   static const char* names[5] = {"vort_col_stage0", "vort_col_stage1", "vort_col_stage2", "vort_col_stage3",
                                  "vort_col_stage4"};
   const int groups = (g.TX + kXCD - 1) / kXCD * kXCD;
+  constexpr int CB = col_cbk<R, N>();
   prof_launch(names[STAGE], col_bytes<R>(g, STAGE, a.do_inverse), kern,
-              dim3(groups * kCSUB), dim3(kCBK * (N / P)), sh, st, g, a);
+              dim3(groups * (kVCB / CB)), dim3(CB * (N / P)), sh, st, g, a);
   FF_LAUNCHED("k_vcol");
   return FFHIP_OK;
 }
@@ -296,8 +318,8 @@ static int launch_rowh(const VortGeom& g, const VortRowArgs& a, hipStream_t st) 
 
 template <typename R, int N>
 static int launch_rowht(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
-  constexpr int M = N / 2, T = 128, Q = M / T;
-  const size_t sh = ((size_t)8 * TeamSched<M, Q>::LDS + tw_slots<M, Q>()) * sizeof(cplx<R>);
+  constexpr int M = N / 2, T = rowht_team<R, N>(), Q = M / T, RPW = rowht_rows<R, N>();
+  const size_t sh = ((size_t)2 * RPW * TeamSched<M, Q>::LDS + tw_slots<M, Q>()) * sizeof(cplx<R>);
   auto kern = k_vrowht<R, N>;
   static bool attr_set = false;
   if (!attr_set) {
@@ -305,8 +327,8 @@ static int launch_rowht(const VortGeom& g, const VortRowArgs& a, hipStream_t st)
                                (int)sh));
     attr_set = true;
   }
-  FF_CHECK(g.ny % 4 == 0, FFHIP_ERR_UNSUPPORTED, "half-length team row kernel: ny = %d", g.ny);
-  prof_launch("vort_row", row_bytes<R>(g), kern, dim3(g.ny / 4), dim3(4 * T), sh, st, g, a);
+  FF_CHECK(g.ny % RPW == 0, FFHIP_ERR_UNSUPPORTED, "half-length team row kernel: ny = %d", g.ny);
+  prof_launch("vort_row", row_bytes<R>(g), kern, dim3(g.ny / RPW), dim3(RPW * T), sh, st, g, a);
   FF_LAUNCHED("k_vrowht");
   return FFHIP_OK;
 }
@@ -319,19 +341,23 @@ static int launch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
   if constexpr (rowht_on<R, N>()) {
     if (rowh_enabled()) return launch_rowht<R, N>(g, a, st);
   }
-  constexpr int P = vrow_pts<R, N>();
-  const size_t sh = row_lds<R, N>();
-  auto kern = k_vrow<R, N, P>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    FF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)sh));
-    attr_set = true;
+  if constexpr (sizeof(R) == 8 && N == 4096)  // the full-length team kernel needs more LDS than a workgroup has
+    return fail(FFHIP_ERR_UNSUPPORTED, "fused FP64 row pass at nx = 4096 needs the half-length kernel (FFHIP_ROWH)");
+  else {
+    constexpr int P = vrow_pts<R, N>();
+    const size_t sh = row_lds<R, N>();
+    auto kern = k_vrow<R, N, P>;
+    static bool attr_set = false;
+    if (!attr_set) {
+      FF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)sh));
+      attr_set = true;
+    }
+    const int groups = (g.ny / 4 + kXCD - 1) / kXCD * kXCD;
+    prof_launch("vort_row", row_bytes<R>(g), kern, dim3(groups * kRSUB), dim3(kRRK * (N / P)), sh, st, g, a);
+    FF_LAUNCHED("k_vrow");
+    return FFHIP_OK;
   }
-  const int groups = (g.ny / 4 + kXCD - 1) / kXCD * kXCD;
-  prof_launch("vort_row", row_bytes<R>(g), kern, dim3(groups * kRSUB), dim3(kRRK * (N / P)), sh, st, g, a);
-  FF_LAUNCHED("k_vrow");
-  return FFHIP_OK;
 }
 
 template <typename R, int STAGE>
@@ -347,9 +373,7 @@ static int dispatch_col(const VortGeom& g, const VortColArgs& a, hipStream_t st)
     case 2048:
       if (col3_on(sizeof(R) == 4)) return launch_col3<R, 2048, STAGE>(g, a, st);
       return launch_col<R, 2048, STAGE>(g, a, st);
-    case 4096:  // FP32 only: an FP64 4096-point column team needs more LDS than a workgroup has
-      if constexpr (sizeof(R) == 4) return launch_col<R, 4096, STAGE>(g, a, st);
-      return fail(FFHIP_ERR_UNSUPPORTED, "fused column kernel: FP64 ny = %d", g.ny);
+    case 4096: return launch_col<R, 4096, STAGE>(g, a, st);
     default: return fail(FFHIP_ERR_UNSUPPORTED, "fused column kernel: ny = %d", g.ny);
   }
 }
@@ -367,9 +391,7 @@ static int dispatch_row(const VortGeom& g, const VortRowArgs& a, hipStream_t st)
     case 2048:
       if (row3_on(sizeof(R) == 4)) return launch_row3<R, 2048>(g, a, st);
       return launch_row<R, 2048>(g, a, st);
-    case 4096:  // FP32 only (two paired 4096-point rows per workgroup: 139 KB of LDS)
-      if constexpr (sizeof(R) == 4) return launch_row<R, 4096>(g, a, st);
-      return fail(FFHIP_ERR_UNSUPPORTED, "fused row kernel: FP64 nx = %d", g.nx);
+    case 4096: return launch_row<R, 4096>(g, a, st);
     default: return fail(FFHIP_ERR_UNSUPPORTED, "fused row kernel: nx = %d", g.nx);
   }
 }
@@ -415,7 +437,7 @@ int fused_create(ffhip_op* op) {
   const char* dis = getenv("FFHIP_DISABLE_FUSED");
   if (dis && atoi(dis) != 0) return FFHIP_OK;
   const int64_t nx = op->n[0], ny = op->n[1];
-  const int64_t nmax = op->prec == FFHIP_F32 ? 4096 : 2048;  // FP32 kernels go to 4096 points
+  const int64_t nmax = 4096;
   if (!pow2(nx) || !pow2(ny) || nx < 64 || ny < 64 || nx > nmax || ny > nmax) return FFHIP_OK;
   if (!(op->af > 0)) return FFHIP_OK;  // the fused kernels assume the Nyquist modes are aliased
   VortGeom g;

@@ -49,7 +49,7 @@ def _oracle(n, ny, Ly, zh0, L, dt, nsteps):
 
 @pytest.mark.parametrize("n,ny,Ly", [(64, 64, 2 * np.pi), (128, 128, 2 * np.pi), (256, 128, 4 * np.pi),
                                      (128, 512, 2 * np.pi), (512, 512, 2 * np.pi), (1024, 1024, 2 * np.pi),
-                                     (2048, 1024, np.pi)])
+                                     (2048, 1024, np.pi), (4096, 64, np.pi / 16), (64, 4096, 32 * np.pi)])
 def test_fused_fp64_matches_oracle(ff, n, ny, Ly):
     o = og.TwoDGrid(n, 2 * np.pi, ny, Ly, x0=0, y0=0)
     zh0 = oics.vorticity_ic(o, seed=11, k0=min(6.0, n / 10))
@@ -114,23 +114,28 @@ def test_fused_three_team_passes(ff, monkeypatch, prec, n, ny, Ly):
     assert err < (1e-12 if prec == 64 else 1e-5), err
 
 
-def test_fused_fp32_4096_square(ff):
-    """4096² FP32 (fused up to 4096 points per axis in FP32; FP64 4096² runs
-    the plan-based path): a few steps against the FP64 plan-based run."""
+def test_fused_4096_square(ff, monkeypatch):
+    """4096² (fused up to 4096 points per axis in both precisions): the FP64
+    fused run against the FP64 plan-based run (the reference's dataflow), and
+    the FP32 fused run against the FP64 one, over a few steps."""
     import torch
 
     n, dt = 4096, 1e-3
     nu8 = oph.hyperviscosity_for(n, dt)
-    p32 = ff.TwoDVorticityProblem(nx=n, nu_n=nu8, n_hyper=4, mu=0.01, dt=dt, T=np.float32)
-    p64 = ff.TwoDVorticityProblem(nx=n, nu_n=nu8, n_hyper=4, mu=0.01, dt=dt, T=np.float64)
-    assert p32.eqn.calcN.fused and not p64.eqn.calcN.fused
+    kw = dict(nx=n, nu_n=nu8, n_hyper=4, mu=0.01, dt=dt)
+    p32 = ff.TwoDVorticityProblem(T=np.float32, **kw)
+    p64 = ff.TwoDVorticityProblem(T=np.float64, **kw)
+    pg = _problem(ff, monkeypatch, False, T=np.float64, **kw)
+    assert p32.eqn.calcN.fused and p64.eqn.calcN.fused and not pg.eqn.calcN.fused
     z0 = ff.vorticity_ic(p64.grid, seed=3)
     p64.sol.copy_(z0)
+    pg.sol.copy_(z0)
     p32.sol.copy_(z0.to(torch.complex64))
-    ff.stepforward(p64, 4)
-    ff.stepforward(p32, 4)
-    err = float(torch.linalg.vector_norm(p32.sol.to(torch.complex128) - p64.sol) / torch.linalg.vector_norm(p64.sol))
-    assert err < 1e-5, err
+    for p in (p64, pg, p32):
+        ff.stepforward(p, 4)
+    rel = lambda a, b: float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))  # noqa: E731
+    assert rel(p64.sol, pg.sol) < 1e-12
+    assert rel(p32.sol.to(torch.complex128), p64.sol) < 1e-5
 
 
 @pytest.mark.parametrize("prec", [64, 32])

@@ -420,11 +420,12 @@ __device__ __forceinline__ void vrow_body(const VortGeom& g, const VortRowArgs& 
 // lds2 are the row's two buffers of TeamSched<M, Q>::LDS slots.
 // Team size T = M/Q: 64 (FP32) is one wavefront, with every twiddle in
 // registers (filled from the NX roots of unity in tw) and wavefront-level
-// exchanges; a larger team (128 threads, two wavefronts) takes its
-// factors from the LDS table ltw, exchanges through workgroup barriers (all
-// rows of the workgroup in step) and fetches the split's mirrored
-// coefficient from LDS instead of another lane.  In the text above read
-// "thread" for "lane" and T for 64 then.
+// exchanges; a larger team (128 or 256 threads, two or four wavefronts)
+// takes its factors from the LDS table ltw, exchanges through workgroup
+// barriers (all rows of the workgroup in step) and fetches the split's
+// mirrored coefficient from LDS instead of another lane.  In the text above
+// read "thread" for "lane" and T for 64 then.  Such a team may pass
+// lds2 = null: its paired transforms then run one after the other in lds1.
 template <typename R, int NX, int Q>
 __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs& a, const int y, const int lane,
                                            cplx<R>* lds1, cplx<R>* lds2, const cplx<R>* ltw = nullptr) {
@@ -453,8 +454,12 @@ __device__ __forceinline__ void vrowh_body(const VortGeom& g, const VortRowArgs&
   auto fft2 = [&](cplx<R>(&x1)[Q], cplx<R>(&x2)[Q], auto dir) {
     if constexpr (WAVE)
       wave_fft2<M, Q, decltype(dir)::value, R>(x1, x2, lds1, lds2, lane, twr);
-    else
+    else if (lds2)
       team_fft2<M, Q, decltype(dir)::value, R>(x1, x2, lds1, lds2, lane, ltw);
+    else {  // one buffer (lds2 null, uniform): the two transforms one after the other
+      team_fft<M, Q, decltype(dir)::value, R>(x1, lds1, lane, ltw);
+      team_fft<M, Q, decltype(dir)::value, R>(x2, lds1, lane, ltw);
+    }
   };
   using Inv = std::integral_constant<int, 1>;
   using Fwd = std::integral_constant<int, -1>;

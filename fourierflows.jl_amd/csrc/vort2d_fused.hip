@@ -175,26 +175,37 @@ template <typename R, int NX>
 constexpr bool rowht_on() {
   return FF_ROWH_TEAM && (NX == 4096 || (sizeof(R) == 8 && NX == 2048));
 }
-// team size and rows per workgroup: FP64 4096-point rows take teams of
-// four wavefronts (2048-point transforms at 8 points per thread), one row
-// per workgroup (102 KB of LDS); the others two-wavefront teams, four rows
+// team size, rows per workgroup and LDS buffers per row: FP64 4096-point
+// rows take teams of four wavefronts (2048-point transforms at 8 points per
+// thread), two rows per workgroup with one buffer each (their paired
+// transforms run one after the other: 102 KB of LDS, two waves per SIMD);
+// the others two-wavefront teams, four rows with two buffers each
+template <typename R, int NX>
+constexpr bool rowht_big() {
+  return sizeof(R) == 8 && NX == 4096;
+}
 template <typename R, int NX>
 constexpr int rowht_team() {
-  return sizeof(R) == 8 && NX == 4096 ? 256 : 128;
+  return rowht_big<R, NX>() ? 256 : 128;
 }
 template <typename R, int NX>
 constexpr int rowht_rows() {
-  return sizeof(R) == 8 && NX == 4096 ? 1 : 4;
+  return rowht_big<R, NX>() ? 2 : 4;
+}
+template <typename R, int NX>
+constexpr int rowht_bufs() {
+  return rowht_big<R, NX>() ? 1 : 2;
 }
 template <typename R, int NX>
 __global__ void __launch_bounds__((rowht_team<R, NX>() * rowht_rows<R, NX>()), 2) k_vrowht(VortGeom g, VortRowArgs a) {
   extern __shared__ unsigned char vsm[];
-  constexpr int M = NX / 2, T = rowht_team<R, NX>(), Q = M / T, RPW = rowht_rows<R, NX>();
+  constexpr int M = NX / 2, T = rowht_team<R, NX>(), Q = M / T, RPW = rowht_rows<R, NX>(), NB = rowht_bufs<R, NX>();
   constexpr int SL = TeamSched<M, Q>::LDS;
   cplx<R>* base = reinterpret_cast<cplx<R>*>(vsm);
-  const cplx<R>* ltw = team_tw_load<M, Q, R, 2>(base + 2 * RPW * SL, a.tw);  // every other NX root
+  const cplx<R>* ltw = team_tw_load<M, Q, R, 2>(base + NB * RPW * SL, a.tw);  // every other NX root
   const int r = threadIdx.x / T, tid = threadIdx.x % T;
-  vrowh_body<R, NX, Q>(g, a, blockIdx.x * RPW + r, tid, base + 2 * r * SL, base + (2 * r + 1) * SL, ltw);
+  cplx<R>* l1 = base + NB * r * SL;
+  vrowh_body<R, NX, Q>(g, a, blockIdx.x * RPW + r, tid, l1, NB == 2 ? l1 + SL : nullptr, ltw);
 }
 
 // ----------------------------------------------------------------- host side
@@ -318,8 +329,8 @@ static int launch_rowh(const VortGeom& g, const VortRowArgs& a, hipStream_t st) 
 
 template <typename R, int N>
 static int launch_rowht(const VortGeom& g, const VortRowArgs& a, hipStream_t st) {
-  constexpr int M = N / 2, T = rowht_team<R, N>(), Q = M / T, RPW = rowht_rows<R, N>();
-  const size_t sh = ((size_t)2 * RPW * TeamSched<M, Q>::LDS + tw_slots<M, Q>()) * sizeof(cplx<R>);
+  constexpr int M = N / 2, T = rowht_team<R, N>(), Q = M / T, RPW = rowht_rows<R, N>(), NB = rowht_bufs<R, N>();
+  const size_t sh = ((size_t)NB * RPW * TeamSched<M, Q>::LDS + tw_slots<M, Q>()) * sizeof(cplx<R>);
   auto kern = k_vrowht<R, N>;
   static bool attr_set = false;
   if (!attr_set) {
